@@ -1,0 +1,206 @@
+/*
+ * s2t.h -- C ABI of the MI355X spectral 2-D turbulence step (libs2t.so).
+ *
+ * This is the boundary that replaces the reference's WebGPU `Solver` / `Fft2D` objects
+ * (src/gpu/solver.js, src/gpu/fft.js). Every entry point is plain C: opaque handles, plain
+ * pointers and sizes, int status (0 = ok, < 0 = error) and a thread-local message from
+ * s2t_last_error(). Host arrays are copied at the call, like the reference's
+ * queue.writeBuffer (src/gpu/solver.js:344, 355). Work is enqueued on the solver's own HIP stream;
+ * only the read and diagnostics calls wait for it.
+ *
+ * Array layout used at this boundary (the fixture layout, reference/make_fixtures.py:11-16 and
+ * src/shaders/solver.wgsl:27-28): a complex N x N field is row-major, index = row*N + col, with
+ * kx = wn(col), ky = wn(row) (fftfreq order, Nyquist at -N/2), stored as interleaved float
+ * (re, im) pairs: 2*N*N floats.
+ */
+#ifndef S2T_H
+#define S2T_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct s2t_solver s2t_solver;
+
+/* status codes */
+#define S2T_OK 0
+#define S2T_EINVAL -1   /* bad argument (size, pointer, count) */
+#define S2T_EHIP -2     /* HIP runtime error */
+#define S2T_ENOMEM -3   /* device allocation failed */
+#define S2T_ESTATE -4   /* call not valid in the solver's current state */
+
+/*
+ * Physical and integration parameters. Mirrors the reference config keys (src/main.js:9-36) as
+ * mapped by Solver.configure (src/gpu/solver.js:306-324) into Params (src/shaders/common.wgsl:8-21).
+ * `viscosity` is the damping rate at the cut-off wavenumber kcut = N/3; the hyperviscosity
+ * coefficient is viscosity / kcut^(2 hyper_p) (solver.js:311). fixed_dt > 0 bypasses the CFL
+ * reduction and uses that step (the reference harness writes dt directly, tests/index.html:188).
+ */
+typedef struct s2t_params {
+  double beta;        /* beta-plane parameter */
+  double drag;        /* linear drag mu */
+  double viscosity;   /* damping rate at kcut (0 = inviscid) */
+  double hyper_p;     /* dissipation order p (1, 2, 4) */
+  double force_amp;   /* stochastic forcing amplitude (0 = off) */
+  double force_k;     /* forcing annulus centre */
+  double force_width; /* forcing annulus width */
+  double cfl;         /* CFL number */
+  double dt_min;      /* lower clamp of the CFL step */
+  double dt_max;      /* upper clamp of the CFL step */
+  double dye_fade;    /* dye persistence per step */
+  double fixed_dt;    /* > 0: use this dt instead of the CFL step */
+} s2t_params;
+
+/* Pointer splat: the record of src/ui/pointer.js:58-67 / Splat in common.wgsl:23-32. */
+typedef struct s2t_splat {
+  float x, y;        /* position in unit coordinates [0, 1) */
+  float dx, dy;      /* pointer velocity (unit coordinates per frame) */
+  float color[3];    /* dye colour */
+  float radius;      /* Gaussian radius (unit coordinates) */
+  float amp;         /* body-force amplitude */
+  float dye_amp;     /* dye injection amplitude */
+} s2t_splat;
+
+#define S2T_MAX_SPLATS 8 /* src/gpu/solver.js:9 */
+
+/* Diagnostics record: the fields of readDiagnostics (solver.js:591-602), in FP64. */
+typedef struct s2t_diag {
+  double energy;     /* 1/2 sum |w|^2/k^2 / N^4 */
+  double enstrophy;  /* 1/2 sum |w|^2 / N^4 */
+  double dt;         /* step used by the last step */
+  double max_speed;  /* max |u|+|v| of the last step's initial field */
+  double time;       /* simulated time */
+  uint32_t step_index; /* number of steps taken since the last reset */
+  uint32_t nonfinite;  /* nonzero once a non-finite velocity was seen */
+} s2t_diag;
+
+/* step flags */
+#define S2T_STEP_DYE 1u /* advect the dye after each step (Solver.encodeDye, solver.js:523-536) */
+
+/* ------------------------------------------------------------------------------------------ */
+/* lifecycle                                                                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Solver.create(device, {n, dyeScale}) (solver.js:71-140). n must be 4^a or 2*4^a with
+ * 16 <= n <= 4096 (fft.js:20-31 gives the error text); dye_scale in {1, 2} (src/main.js:199). */
+int s2t_create(int n, int dye_scale, s2t_solver** out);
+/* Solver.destroy() (solver.js:614-629). Waits for queued work. NULL is accepted. */
+int s2t_destroy(s2t_solver* h);
+/* Thread-local text of the last error on this thread ("" when none). */
+const char* s2t_last_error(void);
+/* radixSchedule(n) (fft.js:20-31): number of radix-4 stages and whether a radix-2 stage
+ * follows; returns S2T_EINVAL with the reference's message for other n. */
+int s2t_radix_schedule(int n, int* radix4_stages, int* has_radix2);
+
+/* ------------------------------------------------------------------------------------------ */
+/* parameters and inputs                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Solver.configure(cfg) (solver.js:306-324). */
+int s2t_configure(s2t_solver* h, const s2t_params* p);
+/* Hyperviscosity coefficient in effect (Solver.paramsF[2], tests/index.html:136-140). */
+int s2t_nu_h(s2t_solver* h, double* nu_h);
+/* Solver.setSplats(list) (solver.js:326-345); count is clamped to S2T_MAX_SPLATS. */
+int s2t_set_splats(s2t_solver* h, const s2t_splat* splats, int count);
+
+/* ------------------------------------------------------------------------------------------ */
+/* seeding (Solver.reset and its two routes, solver.js:347-444)                               */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Solver.seed({k0, amplitude, seed}) via fillSpectrum (solver.js:347-359, 384-418); amplitude 0
+ * gives a fluid at rest. Resets step index, time and dt like Solver.reset (solver.js:420-444). */
+int s2t_seed_spectrum(s2t_solver* h, double k0, double amplitude, uint32_t seed);
+/* Solver.seedPhysical(values) (solver.js:361-382): n*n physical vorticity -> forward FFT ->
+ * 2/3 mask, k = 0 zeroed (src/shaders/seed.wgsl:4-17). Also resets like Solver.reset. */
+int s2t_seed_physical(s2t_solver* h, const float* field);
+/* Write the vorticity spectrum (2*n*n floats, fixture layout). The field must be the spectrum of
+ * a real field; it is stored masked to the 2/3 band with k = 0 zeroed. Step index, time and dt
+ * are left as they are (the reference harness writes state[0] directly, tests/index.html:187). */
+int s2t_write_state(s2t_solver* h, const float* spectrum);
+/* Read the vorticity spectrum (2*n*n floats, fixture layout). Waits for queued work. */
+int s2t_read_state(s2t_solver* h, float* spectrum);
+/* Zero the step index and time and set dt = dt_max (part of Solver.reset). */
+int s2t_reset_clock(s2t_solver* h);
+
+/* ------------------------------------------------------------------------------------------ */
+/* stepping                                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* nsteps x Solver.encodeStep (solver.js:488-521) with the CFL dt of each step taken from its
+ * initial field (reference/spectral2d.py:66-71, 77-78), followed by encodeDye when
+ * (flags & S2T_STEP_DYE). The velocity prime of encodePrime (solver.js:475-486) is implicit.
+ * Enqueues only; does not wait. */
+int s2t_step(s2t_solver* h, int nsteps, uint32_t flags);
+/* Wait for all queued work on the solver's stream. */
+int s2t_sync(s2t_solver* h);
+
+/* ------------------------------------------------------------------------------------------ */
+/* diagnostics and dye                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* readDiagnostics (solver.js:547-608) computed in FP64 with a fixed reduction order:
+ * spectrum[n/2+1] = E(k) in bins round(|k|) (reference/spectral2d.py:98-104), zonal[n] =
+ * zonal-mean u per row (spectral2d.py:106-108). Either array may be NULL. Waits. */
+int s2t_diagnostics(s2t_solver* h, s2t_diag* out, double* spectrum, double* zonal);
+/* Solver.paintDye(pattern) (solver.js:446-473): 0 = 'none', 1 = 'bands'. */
+int s2t_paint_dye(s2t_solver* h, int pattern);
+/* Dye texture side M = n * dye_scale. */
+int s2t_dye_size(s2t_solver* h, int* m);
+/* Read the current dye field (Solver.currentDye) as M*M*4 binary16 values (rgba). Waits. */
+int s2t_read_dye(s2t_solver* h, uint16_t* rgba);
+/* Write the current dye field from M*M*4 binary16 values. */
+int s2t_write_dye(s2t_solver* h, const uint16_t* rgba);
+/* Write the physical velocity (n*n*2 floats, (u, v) per cell) that the dye advection reads;
+ * s2t_step with S2T_STEP_DYE writes it from each step's initial field. */
+int s2t_write_velocity(s2t_solver* h, const float* uv);
+/* One dye update (advect_fwd, advect_back, advect_corr; src/shaders/dye.wgsl:58-113) with the
+ * given dt and the current velocity, splats and fade. */
+int s2t_dye_step(s2t_solver* h, double dt);
+
+/* ------------------------------------------------------------------------------------------ */
+/* test hooks: the product kernels on host data (the reference harness reaches the same stages  */
+/* through Solver internals, tests/index.html:145-201)                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Fft2D.forward / inverse on a full complex n x n field (fft.js:81-100): row pass then column
+ * pass, forward unscaled, inverse scaled by 1/n per pass. in and out may alias. */
+int s2t_fft2(s2t_solver* h, const float* in, float* out, int inverse);
+/* velocity_pack (solver.wgsl:21-40) on a full spectrum: packA = w + i u_hat, packB = v_hat. */
+int s2t_velocity_pack(s2t_solver* h, const float* w, float* pack_a, float* pack_b);
+/* assemble (solver.wgsl:56-95) on a full product spectrum z and pointer-forcing spectrum fhat
+ * (NULL = zero), with the stochastic forcing of the configured amplitude at step index
+ * `step_index` and step `dt`: rhs = (fhat - N_hat) * mask + forcing. */
+int s2t_assemble(s2t_solver* h, const float* z, const float* fhat, uint32_t step_index, double dt,
+                 float* rhs);
+/* Right-hand side -N_hat(w) (+ forcing) of the current state through the fused product path
+ * (row kernel + column kernel), full fixture layout. Uses step index `step_index` and `dt` for
+ * the stochastic forcing. */
+int s2t_rhs(s2t_solver* h, uint32_t step_index, double dt, float* rhs);
+
+/* ------------------------------------------------------------------------------------------ */
+/* profiling: per-kernel HIP-event timing on the solver's stream                                */
+/* ------------------------------------------------------------------------------------------ */
+
+#define S2T_PROF_ROWS 0  /* fused row kernel */
+#define S2T_PROF_COLS 1  /* fused column kernel */
+#define S2T_PROF_DYE 2   /* dye kernels */
+#define S2T_PROF_OTHER 3 /* prime and everything else */
+#define S2T_PROF_KINDS 4
+
+/* Enable (1) or disable (0) event timing around every kernel launched by s2t_step; enabling
+ * clears the totals. */
+int s2t_profile(s2t_solver* h, int enable);
+/* Launch count and summed device time (ms) of one kernel kind since profiling was enabled.
+ * Waits for queued work. */
+int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);
+/* Algorithmic HBM bytes of one launch of a kernel kind for this solver's size and the current
+ * options (stage 0 of a step when the kinds differ by stage). */
+int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S2T_H */

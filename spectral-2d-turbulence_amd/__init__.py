@@ -1,0 +1,303 @@
+"""ctypes binding of libs2t.so (include/s2t.h) for the tests, the bench and smoke checks.
+
+The product host interface is the Node.js N-API addon in ``js/`` (the reference's own host
+language); this module is the same C ABI seen from Python. It loads the in-tree library built by
+``build.py`` and raises if that library is missing: there is no CPU fallback.
+
+Load it by path (the directory name is not a Python identifier)::
+
+    import importlib.util, os
+    spec = importlib.util.spec_from_file_location(
+        "s2t_amd", os.path.join(ROOT, "spectral-2d-turbulence_amd", "__init__.py"))
+    s2t = importlib.util.module_from_spec(spec); spec.loader.exec_module(s2t)
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libs2t.so")
+
+S2T_OK = 0
+S2T_EINVAL = -1
+S2T_EHIP = -2
+S2T_ENOMEM = -3
+S2T_ESTATE = -4
+S2T_STEP_DYE = 1
+PROF_ROWS, PROF_COLS, PROF_DYE, PROF_OTHER = 0, 1, 2, 3
+MAX_SPLATS = 8
+
+# every symbol include/s2t.h declares
+EXPORTS = [
+    "s2t_create", "s2t_destroy", "s2t_last_error", "s2t_radix_schedule", "s2t_configure",
+    "s2t_nu_h", "s2t_set_splats", "s2t_seed_spectrum", "s2t_seed_physical", "s2t_write_state",
+    "s2t_read_state", "s2t_reset_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
+    "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
+    "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
+    "s2t_profile_read", "s2t_kernel_bytes",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in (
+        "beta", "drag", "viscosity", "hyper_p", "force_amp", "force_k", "force_width", "cfl",
+        "dt_min", "dt_max", "dye_fade", "fixed_dt")]
+
+
+class SplatC(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("dx", ctypes.c_float),
+                ("dy", ctypes.c_float), ("color", ctypes.c_float * 3), ("radius", ctypes.c_float),
+                ("amp", ctypes.c_float), ("dye_amp", ctypes.c_float)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("enstrophy", ctypes.c_double),
+                ("dt", ctypes.c_double), ("max_speed", ctypes.c_double),
+                ("time", ctypes.c_double), ("step_index", ctypes.c_uint32),
+                ("nonfinite", ctypes.c_uint32)]
+
+
+class S2TError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+_LIB = None
+
+
+def load(path=None):
+    """Load libs2t.so; raises FileNotFoundError when it has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError("%s is not built: run spectral-2d-turbulence_amd/build.py" % p)
+    lib = ctypes.CDLL(p)
+    vp, ip, fp, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+    u32 = ctypes.c_uint32
+    sig = {
+        "s2t_create": (ip, [ip, ip, ctypes.POINTER(vp)]),
+        "s2t_destroy": (ip, [vp]),
+        "s2t_last_error": (ctypes.c_char_p, []),
+        "s2t_radix_schedule": (ip, [ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]),
+        "s2t_configure": (ip, [vp, ctypes.POINTER(Params)]),
+        "s2t_nu_h": (ip, [vp, ctypes.POINTER(dp)]),
+        "s2t_set_splats": (ip, [vp, ctypes.POINTER(SplatC), ip]),
+        "s2t_seed_spectrum": (ip, [vp, dp, dp, u32]),
+        "s2t_seed_physical": (ip, [vp, fp]),
+        "s2t_write_state": (ip, [vp, fp]),
+        "s2t_read_state": (ip, [vp, fp]),
+        "s2t_reset_clock": (ip, [vp]),
+        "s2t_step": (ip, [vp, ip, u32]),
+        "s2t_sync": (ip, [vp]),
+        "s2t_diagnostics": (ip, [vp, ctypes.POINTER(Diag), fp, fp]),
+        "s2t_paint_dye": (ip, [vp, ip]),
+        "s2t_dye_size": (ip, [vp, ctypes.POINTER(ip)]),
+        "s2t_read_dye": (ip, [vp, fp]),
+        "s2t_write_dye": (ip, [vp, fp]),
+        "s2t_write_velocity": (ip, [vp, fp]),
+        "s2t_dye_step": (ip, [vp, dp]),
+        "s2t_fft2": (ip, [vp, fp, fp, ip]),
+        "s2t_velocity_pack": (ip, [vp, fp, fp, fp]),
+        "s2t_assemble": (ip, [vp, fp, fp, u32, dp, fp]),
+        "s2t_rhs": (ip, [vp, u32, dp, fp]),
+        "s2t_profile": (ip, [vp, ip]),
+        "s2t_profile_read": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp)]),
+        "s2t_kernel_bytes": (ip, [vp, ip, ip, ctypes.POINTER(dp)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _check(lib, rc):
+    if rc != S2T_OK:
+        raise S2TError(rc, lib.s2t_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def to_interleaved(z):
+    """complex N x N -> float32 interleaved (re, im) in the fixture layout."""
+    z = np.asarray(z)
+    out = np.empty(z.shape + (2,), np.float32)
+    out[..., 0] = z.real
+    out[..., 1] = z.imag
+    return np.ascontiguousarray(out.reshape(-1))
+
+
+def from_interleaved(a, n):
+    a = np.asarray(a, np.float64).reshape(n, n, 2)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+DEFAULTS = dict(beta=0.0, drag=0.01, viscosity=0.67, hyper_p=4.0, force_amp=0.0, force_k=22.0,
+                force_width=2.5, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=0.999, fixed_dt=0.0)
+
+
+class Solver:
+    """One solver instance on the current HIP device (mirrors the reference Solver object)."""
+
+    def __init__(self, n, dye_scale=1):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.s2t_create(int(n), int(dye_scale), ctypes.byref(h)))
+        self.h = h
+        self.n = int(n)
+        self.dye_scale = int(dye_scale)
+        self.params = dict(DEFAULTS)
+
+    def close(self):
+        if self.h:
+            self.lib.s2t_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, **kw):
+        unknown = set(kw) - set(DEFAULTS)
+        if unknown:
+            raise KeyError("unknown parameters: %s" % sorted(unknown))
+        self.params.update(kw)
+        p = Params(**{k: float(v) for k, v in self.params.items()})
+        _check(self.lib, self.lib.s2t_configure(self.h, ctypes.byref(p)))
+
+    def nu_h(self):
+        v = ctypes.c_double()
+        _check(self.lib, self.lib.s2t_nu_h(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_splats(self, splats):
+        arr = (SplatC * max(1, len(splats)))()
+        for i, s in enumerate(splats[:MAX_SPLATS]):
+            arr[i].x, arr[i].y, arr[i].dx, arr[i].dy = s["x"], s["y"], s["dx"], s["dy"]
+            c = s.get("color", (0.0, 0.0, 0.0))
+            arr[i].color[0], arr[i].color[1], arr[i].color[2] = c
+            arr[i].radius, arr[i].amp = s["radius"], s["amp"]
+            arr[i].dye_amp = s.get("dyeAmp", 0.0)
+        _check(self.lib, self.lib.s2t_set_splats(self.h, arr, len(splats)))
+
+    def seed_spectrum(self, k0=14.0, amplitude=6.0, seed=1):
+        _check(self.lib, self.lib.s2t_seed_spectrum(self.h, float(k0), float(amplitude),
+                                                    int(seed) & 0xFFFFFFFF))
+
+    def seed_physical(self, field):
+        f = np.ascontiguousarray(np.asarray(field, np.float32).reshape(-1))
+        assert f.size == self.n * self.n
+        _check(self.lib, self.lib.s2t_seed_physical(self.h, _ptr(f)))
+
+    def write_state(self, w):
+        a = to_interleaved(np.asarray(w).reshape(self.n, self.n))
+        _check(self.lib, self.lib.s2t_write_state(self.h, _ptr(a)))
+
+    def read_state(self):
+        a = np.empty(2 * self.n * self.n, np.float32)
+        _check(self.lib, self.lib.s2t_read_state(self.h, _ptr(a)))
+        return from_interleaved(a, self.n)
+
+    def reset_clock(self):
+        _check(self.lib, self.lib.s2t_reset_clock(self.h))
+
+    def step(self, nsteps=1, dye=False):
+        _check(self.lib, self.lib.s2t_step(self.h, int(nsteps), S2T_STEP_DYE if dye else 0))
+
+    def sync(self):
+        _check(self.lib, self.lib.s2t_sync(self.h))
+
+    def diagnostics(self):
+        d = Diag()
+        spec = np.empty(self.n // 2 + 1, np.float64)
+        zonal = np.empty(self.n, np.float64)
+        _check(self.lib, self.lib.s2t_diagnostics(self.h, ctypes.byref(d), _ptr(spec),
+                                                  _ptr(zonal)))
+        return dict(energy=d.energy, enstrophy=d.enstrophy, dt=d.dt, maxSpeed=d.max_speed,
+                    time=d.time, stepIndex=d.step_index, nonfinite=d.nonfinite,
+                    spectrum=spec, zonal=zonal)
+
+    def paint_dye(self, pattern="bands"):
+        _check(self.lib, self.lib.s2t_paint_dye(self.h, {"none": 0, "bands": 1}[pattern]))
+
+    def dye_size(self):
+        m = ctypes.c_int()
+        _check(self.lib, self.lib.s2t_dye_size(self.h, ctypes.byref(m)))
+        return m.value
+
+    def read_dye(self):
+        m = self.dye_size()
+        a = np.empty(m * m * 4, np.uint16)
+        _check(self.lib, self.lib.s2t_read_dye(self.h, _ptr(a)))
+        return a.view(np.float16).reshape(m, m, 4)
+
+    def write_dye(self, rgba):
+        a = np.ascontiguousarray(np.asarray(rgba, np.float16)).view(np.uint16).reshape(-1)
+        _check(self.lib, self.lib.s2t_write_dye(self.h, _ptr(a)))
+
+    def write_velocity(self, uv):
+        a = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1))
+        _check(self.lib, self.lib.s2t_write_velocity(self.h, _ptr(a)))
+
+    def dye_step(self, dt):
+        _check(self.lib, self.lib.s2t_dye_step(self.h, float(dt)))
+
+    # test hooks
+    def fft2(self, z, inverse=False):
+        a = to_interleaved(np.asarray(z).reshape(self.n, self.n))
+        out = np.empty_like(a)
+        _check(self.lib, self.lib.s2t_fft2(self.h, _ptr(a), _ptr(out), 1 if inverse else 0))
+        return from_interleaved(out, self.n)
+
+    def velocity_pack(self, w):
+        a = to_interleaved(np.asarray(w).reshape(self.n, self.n))
+        pa = np.empty_like(a)
+        pb = np.empty_like(a)
+        _check(self.lib, self.lib.s2t_velocity_pack(self.h, _ptr(a), _ptr(pa), _ptr(pb)))
+        return from_interleaved(pa, self.n), from_interleaved(pb, self.n)
+
+    def assemble(self, z, fhat=None, step_index=0, dt=0.01):
+        a = to_interleaved(np.asarray(z).reshape(self.n, self.n))
+        f = None if fhat is None else to_interleaved(np.asarray(fhat).reshape(self.n, self.n))
+        out = np.empty_like(a)
+        _check(self.lib, self.lib.s2t_assemble(self.h, _ptr(a), None if f is None else _ptr(f),
+                                               int(step_index), float(dt), _ptr(out)))
+        return from_interleaved(out, self.n)
+
+    def rhs(self, step_index=0, dt=0.01):
+        out = np.empty(2 * self.n * self.n, np.float32)
+        _check(self.lib, self.lib.s2t_rhs(self.h, int(step_index), float(dt), _ptr(out)))
+        return from_interleaved(out, self.n)
+
+    # profiling
+    def profile(self, enable):
+        _check(self.lib, self.lib.s2t_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, kind):
+        n = ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(self.lib, self.lib.s2t_profile_read(self.h, kind, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def kernel_bytes(self, kind, stage=0):
+        b = ctypes.c_double()
+        _check(self.lib, self.lib.s2t_kernel_bytes(self.h, kind, stage, ctypes.byref(b)))
+        return b.value

@@ -1,0 +1,25 @@
+// Host launchers of the kernels in aux_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "physics.hpp"
+#include "solver.hpp"
+
+namespace s2t {
+
+void fill_u32(hipStream_t st, void* p, size_t words, uint32_t v);
+void reset_clock(hipStream_t st, DevState* d, double dt_max);
+void vpack_full(hipStream_t st, const float2* w, float2* pa, float2* pb, int n);
+void assemble_full(hipStream_t st, const float2* z, const float2* fhat, float2* rhs,
+                   const Phys& ph, uint32_t step, double dt, int n);
+void real_to_complex(hipStream_t st, const float* r, float2* c, size_t n);
+void seed_spectrum(hipStream_t st, float2* state, double* scratch, int n, double k0,
+                   double amplitude, uint32_t seed);
+void diagnostics(hipStream_t st, const float2* state, const double2* wtab, double* bins,
+                 double* zonal, int n);
+void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
+              const DevState* dev, double dt, float fade, const Splats& sp, int n, int m);
+void dye_paint(hipStream_t st, ushort4* dst, int m);
+
+}  // namespace s2t

@@ -1,0 +1,491 @@
+// Kernels off the fused step path: seeding, FP64 diagnostics, BFECC dye and the full-field
+// test hooks. Grid size is a runtime argument here (no line FFTs).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "aux.hpp"
+#include "physics.hpp"
+
+namespace s2t {
+
+// compact-state geometry at runtime
+struct RGeo {
+  int n, K, KC, NJ, pitch;
+  __host__ __device__ int jrow(int i) const {
+    return i <= K ? i : (i >= n - K ? i - (n - NJ) : -1);
+  }
+};
+static RGeo rgeo(int n) {
+  RGeo g;
+  g.n = n;
+  g.K = n / 3;
+  g.KC = g.K + 1;
+  g.NJ = 2 * g.K + 1;
+  g.pitch = (g.NJ + 63) / 64 * 64;
+  return g;
+}
+
+static unsigned blocks(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------------------------
+// fill kernels (used instead of memset so a captured graph never holds a memset node)
+// ---------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill_u32(hipStream_t st, void* p, size_t words, uint32_t v) {
+  if (words == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(words, 256)), dim3(256), 0, st, (uint32_t*)p, words,
+                     v);
+  check_launch("k_fill_u32");
+}
+
+__global__ void k_reset_clock(DevState* d, double dt_max) {
+  d->dt = dt_max;
+  d->t = 0.0;
+  d->speed = 0.f;
+  d->speed_bits = 0u;
+  d->step = 0xFFFFFFFFu;
+  d->nonfinite = 0u;
+}
+void reset_clock(hipStream_t st, DevState* d, double dt_max) {
+  hipLaunchKernelGGL(k_reset_clock, dim3(1), dim3(1), 0, st, d, dt_max);
+  check_launch("k_reset_clock");
+}
+
+// ---------------------------------------------------------------------------------------------
+// test hooks on full N x N fields
+// ---------------------------------------------------------------------------------------------
+// velocity_pack (solver.wgsl:21-40)
+__global__ void k_vpack_full(const float2* __restrict__ w, float2* __restrict__ pa,
+                             float2* __restrict__ pb, int n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * n) return;
+  const float kx = (float)wn((int)(i % n), n), ky = (float)wn((int)(i / n), n);
+  const float k2 = kx * kx + ky * ky;
+  const float inv = k2 > 0.f ? 1.f / k2 : 0.f;
+  const float2 v = w[i];
+  float2 uh, vh;
+  vel_pack(v, kx, ky, inv, uh, vh);
+  pa[i] = cadd(v, cmuli(uh));
+  pb[i] = vh;
+}
+void vpack_full(hipStream_t st, const float2* w, float2* pa, float2* pb, int n) {
+  hipLaunchKernelGGL(k_vpack_full, dim3(blocks((size_t)n * n, 256)), dim3(256), 0, st, w, pa, pb,
+                     n);
+  check_launch("k_vpack_full");
+}
+
+// assemble (solver.wgsl:56-95)
+__global__ void k_assemble_full(const float2* __restrict__ z, const float2* __restrict__ fhat,
+                                float2* __restrict__ rhs, Phys ph, uint32_t step, float dt,
+                                int n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * n) return;
+  const int col = (int)(i % n), row = (int)(i / n);
+  const float kx = (float)wn(col, n), ky = (float)wn(row, n);
+  const float k2 = kx * kx + ky * ky;
+  const float2 zk = z[i];
+  const float2 zm = z[(size_t)((n - row) % n) * n + (n - col) % n];
+  float2 q1, q2;
+  herm_unpack(zk, zm, q1, q2);
+  const float2 nh = nhat(q1, q2, kx, ky);
+  const float kcut = (float)n / 3.f;
+  const bool band = fabsf(kx) < kcut && fabsf(ky) < kcut;
+  float2 f = make_float2(0.f, 0.f);
+  if (band) {
+    const float2 fh = fhat ? fhat[i] : make_float2(0.f, 0.f);
+    f = csub(fh, nh);
+    if (ph.force_amp > 0.f && k2 > 0.f) f = cadd(f, stoch_force(row, col, kx, ky, k2, ph, step, dt, n));
+  }
+  rhs[i] = f;
+}
+void assemble_full(hipStream_t st, const float2* z, const float2* fhat, float2* rhs,
+                   const Phys& ph, uint32_t step, double dt, int n) {
+  hipLaunchKernelGGL(k_assemble_full, dim3(blocks((size_t)n * n, 256)), dim3(256), 0, st, z, fhat,
+                     rhs, ph, step, (float)dt, n);
+  check_launch("k_assemble_full");
+}
+
+// real physical field -> complex (imaginary part zero)
+__global__ void k_real_to_complex(const float* __restrict__ r, float2* __restrict__ c, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = make_float2(r[i], 0.f);
+}
+void real_to_complex(hipStream_t st, const float* r, float2* c, size_t n) {
+  hipLaunchKernelGGL(k_real_to_complex, dim3(blocks(n, 256)), dim3(256), 0, st, r, c, n);
+  check_launch("k_real_to_complex");
+}
+
+// ---------------------------------------------------------------------------------------------
+// seeding: fillSpectrum (src/gpu/solver.js:384-418) on the device, FP64 like the JS host loop
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double seed_amp(int kx, int ky, double k0) {
+  const double kr = hypot((double)kx, (double)ky);
+  const double env = (kr * kr) / (1.0 + pow(kr / k0, 8.0));
+  return sqrt(env);
+}
+
+// per-column sum of amp^2 over the stored modes (k = 0 excluded)
+__global__ void k_seed_colsum(RGeo g, double k0, double* __restrict__ colsum) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kx >= g.KC) return;
+  double s = 0.0;
+  for (int j = 0; j < g.NJ; ++j) {
+    const int ky = j <= g.K ? j : j - g.NJ;
+    if (kx == 0 && ky == 0) continue;
+    const double a = seed_amp(kx, ky, k0);
+    s += a * a;
+  }
+  colsum[kx] = kx == 0 ? s : 2.0 * s;  // kx > 0 columns also stand for their mirror
+}
+
+// scale = amplitude / (sqrt(sum)/N^2), or 0 when the sum is 0; one block, fixed order
+__global__ void k_seed_scale(RGeo g, double amplitude, double* __restrict__ colsum) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < g.KC; i += 256) s += colsum[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(sh[0]) / ((double)g.n * (double)g.n);
+    colsum[g.KC] = rms > 0.0 ? amplitude / rms : 0.0;
+  }
+}
+
+__global__ void k_seed_fill(RGeo g, double k0, uint32_t salt, const double* __restrict__ scale,
+                            float2* __restrict__ st) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)g.KC * g.pitch) return;
+  const int kx = (int)(i / g.pitch), j = (int)(i % g.pitch);
+  float2 v = make_float2(0.f, 0.f);
+  const int ky = j <= g.K ? j : j - g.NJ;
+  if (j < g.NJ && !(kx == 0 && ky == 0)) {
+    const int n = g.n;
+    const int row = (ky + n) % n, col = kx;
+    const double amp = seed_amp(kx, ky, k0);
+    const bool canon = ky > 0 || (ky == 0 && kx > 0);
+    const int hr = canon ? row : (n - row) % n;
+    const int hc = canon ? col : (n - col) % n;
+    const double sgn = canon ? 1.0 : -1.0;
+    const double phase = 2.0 * M_PI * (double)hash1((uint32_t)(hr * n + hc) ^ salt) / 4294967296.0;
+    float re = (float)(amp * cos(phase));
+    float im = (float)(sgn * amp * sin(phase));
+    const double sc = *scale;
+    if (sc > 0.0) {
+      re = (float)((double)re * sc);
+      im = (float)((double)im * sc);
+    }
+    v = make_float2(re, im);
+  }
+  st[i] = v;
+}
+
+void seed_spectrum(hipStream_t stm, float2* st, double* scratch, int n, double k0, double amplitude,
+                   uint32_t seed) {
+  const RGeo g = rgeo(n);
+  if (amplitude > 0.0) {
+    hipLaunchKernelGGL(k_seed_colsum, dim3(blocks(g.KC, 64)), dim3(64), 0, stm, g, k0, scratch);
+    check_launch("k_seed_colsum");
+    hipLaunchKernelGGL(k_seed_scale, dim3(1), dim3(256), 0, stm, g, amplitude, scratch);
+    check_launch("k_seed_scale");
+    hipLaunchKernelGGL(k_seed_fill, dim3(blocks((size_t)g.KC * g.pitch, 256)), dim3(256), 0, stm,
+                       g, k0, hash1(seed), scratch + g.KC, st);
+    check_launch("k_seed_fill");
+  } else {
+    fill_u32(stm, st, (size_t)g.KC * g.pitch * 2, 0u);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// diagnostics in FP64 with a fixed reduction order (readDiagnostics, solver.js:547-608;
+// reference/spectral2d.py:92-108)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int isqrt_floor(long v) {
+  if (v <= 0) return 0;
+  long r = (long)sqrt((double)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return (int)r;
+}
+
+// block b sums |w|^2 (weight 2 for kx > 0, the mirror half) over modes with round(|k|) = b,
+// i.e. b^2 - b < k^2 <= b^2 + b (|k| is never a half integer)
+__global__ void k_diag_bins(RGeo g, const float2* __restrict__ st, double* __restrict__ be,
+                            double* __restrict__ bz) {
+  __shared__ double she[256], shz[256];
+  const int b = blockIdx.x;
+  const long lo = b == 0 ? 0 : (long)b * b - b + 1, hi = (long)b * b + b;
+  double se = 0.0, sz = 0.0;
+  const int kxmax = min(g.K, b);
+  for (int kx = threadIdx.x; kx <= kxmax; kx += 256) {
+    const long kx2 = (long)kx * kx;
+    const long a = lo - kx2, c = hi - kx2;
+    if (c < 0) continue;
+    int ky0 = a <= 0 ? 0 : isqrt_floor(a - 1) + 1;
+    int ky1 = min(isqrt_floor(c), g.K);
+    const double wgt = kx == 0 ? 1.0 : 2.0;
+    for (int ky = ky0; ky <= ky1; ++ky) {
+      const double k2 = (double)(kx2 + (long)ky * ky);
+      for (int sgn = 0; sgn < (ky == 0 ? 1 : 2); ++sgn) {
+        const int kys = sgn ? -ky : ky;
+        const int j = kys >= 0 ? kys : kys + g.NJ;
+        const float2 w = st[(size_t)kx * g.pitch + j];
+        const double m2 = (double)w.x * w.x + (double)w.y * w.y;
+        sz += wgt * m2;
+        if (k2 > 0.0) se += wgt * m2 / k2;
+      }
+    }
+  }
+  she[threadIdx.x] = se;
+  shz[threadIdx.x] = sz;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      she[threadIdx.x] += she[threadIdx.x + w];
+      shz[threadIdx.x] += shz[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n4 = (double)g.n * g.n * (double)g.n * g.n;
+    be[b] = 0.5 * she[0] / n4;
+    bz[b] = 0.5 * shz[0] / n4;
+  }
+}
+
+__global__ void k_diag_totals(int nbins, const double* __restrict__ be,
+                              const double* __restrict__ bz, double* __restrict__ out) {
+  __shared__ double she[256], shz[256];
+  double se = 0.0, sz = 0.0;
+  for (int i = threadIdx.x; i < nbins; i += 256) {
+    se += be[i];
+    sz += bz[i];
+  }
+  she[threadIdx.x] = se;
+  shz[threadIdx.x] = sz;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      she[threadIdx.x] += she[threadIdx.x + w];
+      shz[threadIdx.x] += shz[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = she[0];
+    out[1] = shz[0];
+  }
+}
+
+// zonal mean of u in row y: (1/N^2) Re sum_ky u_hat(0, ky) exp(2 pi i ky y / N)
+__global__ void k_diag_zonal(RGeo g, const float2* __restrict__ st,
+                             const double2* __restrict__ wtab, double* __restrict__ zonal) {
+  __shared__ double sh[256];
+  const int y = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < g.NJ; j += 256) {
+    const int ky = j <= g.K ? j : j - g.NJ;
+    if (ky == 0) continue;
+    const float2 w = st[j];
+    // u_hat = i ky w / ky^2 = i w / ky
+    const double ur = -(double)w.y / ky, ui = (double)w.x / ky;
+    const long m = (((long)ky * y) % g.n + g.n) % g.n;
+    const double2 e = wtab[m];
+    s += ur * e.x - ui * e.y;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) zonal[y] = sh[0] / ((double)g.n * (double)g.n);
+}
+
+void diagnostics(hipStream_t stm, const float2* st, const double2* wtab, double* bins,
+                 double* zonal, int n) {
+  const RGeo g = rgeo(n);
+  const int nb = n / 2 + 1;
+  hipLaunchKernelGGL(k_diag_bins, dim3(nb), dim3(256), 0, stm, g, st, bins, bins + nb);
+  check_launch("k_diag_bins");
+  hipLaunchKernelGGL(k_diag_totals, dim3(1), dim3(256), 0, stm, nb, bins, bins + nb,
+                     bins + 2 * nb);
+  check_launch("k_diag_totals");
+  hipLaunchKernelGGL(k_diag_zonal, dim3(n), dim3(256), 0, stm, g, st, wtab, zonal);
+  check_launch("k_diag_zonal");
+}
+
+// ---------------------------------------------------------------------------------------------
+// dye: BFECC with MacCormack limiter (src/shaders/dye.wgsl:1-113) on binary16 rgba texels with
+// FP32 bilinear filtering and repeat addressing
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float4 texel(const ushort4* __restrict__ tex, int m, int ix, int iy) {
+  const int x = ((ix % m) + m) % m, y = ((iy % m) + m) % m;
+  const ushort4 v = tex[(size_t)y * m + x];
+  return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
+                     __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
+}
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
+  return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
+                     a.w + (b.w - a.w) * f);
+}
+__device__ __forceinline__ float4 sample_tex(const ushort4* __restrict__ tex, int m, float px,
+                                             float py) {
+  const float gx = px * (float)m - 0.5f, gy = py * (float)m - 0.5f;
+  const float bx = floorf(gx), by = floorf(gy);
+  const float fx = gx - bx, fy = gy - by;
+  const int ix = (int)bx, iy = (int)by;
+  const float4 top = lerp4(texel(tex, m, ix, iy), texel(tex, m, ix + 1, iy), fx);
+  const float4 bot = lerp4(texel(tex, m, ix, iy + 1), texel(tex, m, ix + 1, iy + 1), fx);
+  return lerp4(top, bot, fy);
+}
+__device__ __forceinline__ float2 vel_at(const float2* __restrict__ vel, int n, int ix, int iy) {
+  const int x = ((ix % n) + n) % n, y = ((iy % n) + n) % n;
+  return vel[(size_t)y * n + x];
+}
+__device__ __forceinline__ float2 sample_vel(const float2* __restrict__ vel, int n, float px,
+                                             float py) {
+  const float gx = px * (float)n - 0.5f, gy = py * (float)n - 0.5f;
+  const float bx = floorf(gx), by = floorf(gy);
+  const float fx = gx - bx, fy = gy - by;
+  const int ix = (int)bx, iy = (int)by;
+  const float2 v00 = vel_at(vel, n, ix, iy), v10 = vel_at(vel, n, ix + 1, iy);
+  const float2 v01 = vel_at(vel, n, ix, iy + 1), v11 = vel_at(vel, n, ix + 1, iy + 1);
+  const float2 a = make_float2(v00.x + (v10.x - v00.x) * fx, v00.y + (v10.y - v00.y) * fx);
+  const float2 b = make_float2(v01.x + (v11.x - v01.x) * fx, v01.y + (v11.y - v01.y) * fx);
+  return make_float2(a.x + (b.x - a.x) * fy, a.y + (b.y - a.y) * fy);
+}
+__device__ __forceinline__ float fract1(float x) { return x - floorf(x); }
+// RK2 midpoint back-trace (dye.wgsl:40-45)
+__device__ __forceinline__ float2 trace(const float2* __restrict__ vel, int n, float px, float py,
+                                        float h) {
+  const float2 v1 = sample_vel(vel, n, px, py);
+  const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
+  const float2 v2 = sample_vel(vel, n, mx, my);
+  return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+}
+__device__ __forceinline__ ushort4 to_half4(float4 v) {
+  return make_ushort4(__half_as_ushort(__float2half(v.x)), __half_as_ushort(__float2half(v.y)),
+                      __half_as_ushort(__float2half(v.z)), __half_as_ushort(__float2half(v.w)));
+}
+
+__device__ __forceinline__ double dye_dt(const DevState* dev, double dt) {
+  return dev ? dev->dt : dt;
+}
+
+// advect_fwd (sign +1) and advect_back (sign -1), dye.wgsl:58-74
+__global__ void k_dye_advect(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
+                             const float2* __restrict__ vel, const DevState* dev, double dt,
+                             float sign, int n, int m) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= m || y >= m) return;
+  const float h = sign * (float)dye_dt(dev, dt) / TWO_PI_F;
+  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+  const float2 s = trace(vel, n, px, py, h);
+  dst[(size_t)y * m + x] = to_half4(sample_tex(src, m, s.x, s.y));
+}
+
+// advect_corr, dye.wgsl:76-113
+__global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __restrict__ aux,
+                           ushort4* __restrict__ dst, const float2* __restrict__ vel,
+                           const DevState* dev, double dt, float fade, Splats sp, int n, int m) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= m || y >= m) return;
+  const float h = (float)dye_dt(dev, dt) / TWO_PI_F;
+  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+  const float2 s = trace(vel, n, px, py, h);
+  const float4 a = sample_tex(src, m, s.x, s.y);
+  const float4 b = sample_tex(aux, m, s.x, s.y);
+  float4 corr = make_float4(1.5f * a.x - 0.5f * b.x, 1.5f * a.y - 0.5f * b.y,
+                            1.5f * a.z - 0.5f * b.z, 1.5f * a.w - 0.5f * b.w);
+  const float gx = s.x * (float)m - 0.5f, gy = s.y * (float)m - 0.5f;
+  const int ix = (int)floorf(gx), iy = (int)floorf(gy);
+  const float4 c00 = texel(src, m, ix, iy), c10 = texel(src, m, ix + 1, iy);
+  const float4 c01 = texel(src, m, ix, iy + 1), c11 = texel(src, m, ix + 1, iy + 1);
+  const float4 lo = make_float4(fminf(fminf(c00.x, c10.x), fminf(c01.x, c11.x)),
+                                fminf(fminf(c00.y, c10.y), fminf(c01.y, c11.y)),
+                                fminf(fminf(c00.z, c10.z), fminf(c01.z, c11.z)),
+                                fminf(fminf(c00.w, c10.w), fminf(c01.w, c11.w)));
+  const float4 hi = make_float4(fmaxf(fmaxf(c00.x, c10.x), fmaxf(c01.x, c11.x)),
+                                fmaxf(fmaxf(c00.y, c10.y), fmaxf(c01.y, c11.y)),
+                                fmaxf(fmaxf(c00.z, c10.z), fmaxf(c01.z, c11.z)),
+                                fmaxf(fmaxf(c00.w, c10.w), fmaxf(c01.w, c11.w)));
+  float4 res = make_float4(fminf(fmaxf(corr.x, lo.x), hi.x) * fade,
+                           fminf(fmaxf(corr.y, lo.y), hi.y) * fade,
+                           fminf(fmaxf(corr.z, lo.z), hi.z) * fade,
+                           fminf(fmaxf(corr.w, lo.w), hi.w) * fade);
+  for (int i = 0; i < sp.count; ++i) {
+    const Splat q = sp.s[i];
+    const float rx = sep(px, q.x), ry = sep(py, q.y);
+    const float sig2 = fmaxf(q.radius * q.radius, 1e-8f);
+    const float r2 = rx * rx + ry * ry;
+    if (r2 < 9.f * sig2) {
+      const float w = q.dye_amp * expf(-r2 / sig2);
+      res.x += q.r * w;
+      res.y += q.g * w;
+      res.z += q.b * w;
+    }
+  }
+  res.w = 1.f;
+  dst[(size_t)y * m + x] = to_half4(res);
+}
+
+void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
+              const DevState* dev, double dt, float fade, const Splats& sp, int n, int m) {
+  const dim3 blk(16, 16), grd((m + 15) / 16, (m + 15) / 16);
+  const int p = parity;
+  // bind-group parity of src/gpu/solver.js:299-303
+  hipLaunchKernelGGL(k_dye_advect, grd, blk, 0, st, dye[p], dye[1 - p], vel, dev, dt, 1.f, n, m);
+  check_launch("k_dye_advect(fwd)");
+  hipLaunchKernelGGL(k_dye_advect, grd, blk, 0, st, dye[1 - p], dye[2], vel, dev, dt, -1.f, n, m);
+  check_launch("k_dye_advect(back)");
+  hipLaunchKernelGGL(k_dye_corr, grd, blk, 0, st, dye[p], dye[2], dye[1 - p], vel, dev, dt, fade,
+                     sp, n, m);
+  check_launch("k_dye_corr");
+}
+
+// toHalf of src/gpu/solver.js:20-41 (mantissa truncation)
+__device__ __forceinline__ uint16_t to_half_trunc(float value) {
+  const uint32_t x = __float_as_uint(value);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  int e = (int)((x >> 23) & 0xffu);
+  uint32_t mant = x & 0x7fffffu;
+  if (e == 255) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+  e = e - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant = (mant | 0x800000u) >> (1 - e);
+    return (uint16_t)(sign | (mant >> 13));
+  }
+  return (uint16_t)(sign | ((uint32_t)e << 10) | (mant >> 13));
+}
+
+// paintDye('bands') of src/gpu/solver.js:446-473, FP64 like the JS loop
+__global__ void k_dye_paint(ushort4* __restrict__ dst, int m) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= m || y >= m) return;
+  const double fy = (double)y / m, fx = (double)x / m;
+  const double band = 0.5 + 0.5 * tanh(6.0 * sin(fy * M_PI * 12.0));
+  const double mix = 0.5 + 0.5 * sin(fx * M_PI * 2.0);
+  const double warm[3] = {1.0, 0.45, 0.22}, cool[3] = {0.28, 0.62, 1.0};
+  uint16_t c[3];
+  for (int k = 0; k < 3; ++k)
+    c[k] = to_half_trunc((float)(band * (warm[k] * mix + cool[k] * (1.0 - mix)) * 0.9));
+  dst[(size_t)y * m + x] = make_ushort4(c[0], c[1], c[2], to_half_trunc(1.0f));
+}
+void dye_paint(hipStream_t st, ushort4* dst, int m) {
+  const dim3 blk(16, 16), grd((m + 15) / 16, (m + 15) / 16);
+  hipLaunchKernelGGL(k_dye_paint, grd, blk, 0, st, dst, m);
+  check_launch("k_dye_paint");
+}
+
+}  // namespace s2t

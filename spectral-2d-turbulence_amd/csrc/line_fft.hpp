@@ -1,0 +1,227 @@
+// Workgroup line FFT for gfx950: register radix-R butterflies, Stockham exchanges through LDS.
+//
+// Replaces the in-place radix-4 DIT of src/shaders/fft.wgsl:42-79 (one workgroup per line,
+// digit-reversal scatter on load, one LDS round trip per radix-4 stage, FP32 twiddles from
+// cos/sin each butterfly). Here a line of N complex values is held as N/T values per thread in
+// registers; each pass does one radix-R (R <= 16) DFT per thread in registers and the passes
+// exchange through LDS, so a 4096-point line needs 3 passes and 2 LDS exchanges instead of 6.
+// The Stockham (autosort) order needs no digit reversal: thread t holds x[t + q*T] on entry and
+// X[t + q*T] on exit, so loads and stores of a contiguous line are coalesced.
+//
+// Conventions match the reference: forward exp(-2 pi i jk/N) unscaled; the inverse transform is
+// exp(+2 pi i jk/N) and is left unscaled here (callers fold the 1/N per pass of
+// src/gpu/fft.js:59 into an exact power-of-two scale elsewhere).
+//
+// Twiddles between passes come from a table computed in FP64 on the host and rounded once to
+// FP32 (table layout in twiddle_table(); inverse uses the conjugate).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace s2t {
+
+// ---------------------------------------------------------------------------------------------
+// complex helpers (float2 = interleaved re, im, the layout of the reference's vec2<f32>)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by i
+__device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x); }
+
+// multiply by (DIR * i): DIR = -1 forward (-i), +1 inverse (+i)
+template <int DIR>
+__device__ __forceinline__ float2 mul_dir_i(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// multiply by (c, DIR*s)
+template <int DIR>
+__device__ __forceinline__ float2 mul_cs(float2 a, float c, float s) {
+  const float sd = DIR < 0 ? -s : s;
+  return make_float2(a.x * c - a.y * sd, a.x * sd + a.y * c);
+}
+
+// cos/sin of k*pi/8 rounded once from their FP64 values
+constexpr float C8_1 = 0.92387953251128675613f;  // cos(pi/8)
+constexpr float S8_1 = 0.38268343236508977173f;  // sin(pi/8)
+constexpr float RSQRT2 = 0.70710678118654752440f;
+
+// ---------------------------------------------------------------------------------------------
+// in-register DFTs, natural-order in and out
+// ---------------------------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  const float2 t = csub(a, b);
+  a = cadd(a, b);
+  b = t;
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  const float2 s13 = cadd(a1, a3), d13 = mul_dir_i<DIR>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a1 = cadd(d02, d13);
+  a2 = csub(s02, s13);
+  a3 = csub(d02, d13);
+}
+
+// W8^1 = (1, DIR)/sqrt2 and W8^3 = (-1, DIR)/sqrt2
+template <int DIR>
+__device__ __forceinline__ float2 mul_w8_1(float2 a) {
+  return DIR < 0 ? make_float2((a.x + a.y) * RSQRT2, (a.y - a.x) * RSQRT2)
+                 : make_float2((a.x - a.y) * RSQRT2, (a.y + a.x) * RSQRT2);
+}
+template <int DIR>
+__device__ __forceinline__ float2 mul_w8_3(float2 a) {
+  return DIR < 0 ? make_float2((a.y - a.x) * RSQRT2, -(a.x + a.y) * RSQRT2)
+                 : make_float2(-(a.x + a.y) * RSQRT2, (a.x - a.y) * RSQRT2);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* v) {
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  o1 = mul_w8_1<DIR>(o1);
+  o2 = mul_dir_i<DIR>(o2);
+  o3 = mul_w8_3<DIR>(o3);
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+// 16 = 4 x 4: DFT4 over n1 for each n2, twiddle W16^(n2 k1), DFT4 over n2, transpose.
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* v) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4<DIR>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+  // slot n2 + 4 k1 holds Y[n2][k1]
+  v[1 + 4 * 1] = mul_cs<DIR>(v[1 + 4 * 1], C8_1, S8_1);     // W^1
+  v[1 + 4 * 2] = mul_w8_1<DIR>(v[1 + 4 * 2]);               // W^2
+  v[1 + 4 * 3] = mul_cs<DIR>(v[1 + 4 * 3], S8_1, C8_1);     // W^3
+  v[2 + 4 * 1] = mul_w8_1<DIR>(v[2 + 4 * 1]);               // W^2
+  v[2 + 4 * 2] = mul_dir_i<DIR>(v[2 + 4 * 2]);              // W^4
+  v[2 + 4 * 3] = mul_w8_3<DIR>(v[2 + 4 * 3]);               // W^6
+  v[3 + 4 * 1] = mul_cs<DIR>(v[3 + 4 * 1], S8_1, C8_1);     // W^3
+  v[3 + 4 * 2] = mul_w8_3<DIR>(v[3 + 4 * 2]);               // W^6
+  v[3 + 4 * 3] = mul_cs<DIR>(v[3 + 4 * 3], -C8_1, -S8_1);   // W^9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<DIR>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+  // slot 4 k1 + k2 holds X[k1 + 4 k2]: transpose the 4 x 4 block
+  float2 t[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) t[i] = v[i];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[4 * k1 + k2];
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else if constexpr (R == 16) dft16<DIR>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ---------------------------------------------------------------------------------------------
+// radix plans. N must be 4^a or 2*4^a (src/gpu/fft.js:20-31); the first radix is the largest,
+// so a thread holds PTS = R0 values and T = N / R0 threads cooperate on one line.
+// ---------------------------------------------------------------------------------------------
+template <int N> struct Plan;
+template <> struct Plan<16>   { static constexpr int R0 = 4,  R1 = 4,  R2 = 1; };
+template <> struct Plan<32>   { static constexpr int R0 = 8,  R1 = 4,  R2 = 1; };
+template <> struct Plan<64>   { static constexpr int R0 = 8,  R1 = 8,  R2 = 1; };
+template <> struct Plan<128>  { static constexpr int R0 = 16, R1 = 8,  R2 = 1; };
+template <> struct Plan<256>  { static constexpr int R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Plan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
+template <> struct Plan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4; };
+template <> struct Plan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8; };
+template <> struct Plan<4096> { static constexpr int R0 = 16, R1 = 16, R2 = 16; };
+
+template <int N>
+struct Line {
+  static constexpr int R0 = Plan<N>::R0, R1 = Plan<N>::R1, R2 = Plan<N>::R2;
+  static constexpr int PTS = R0;       // values per thread
+  static constexpr int T = N / R0;     // threads per line
+  static constexpr int NPASS = R2 > 1 ? 3 : 2;
+  static constexpr int TW1 = 0;                              // pass-1 table offset
+  static constexpr int TW2 = (R1 - 1) * R0;                  // pass-2 table offset
+  static constexpr int TWSIZE = TW2 + (R2 > 1 ? (R2 - 1) * R0 * R1 : 0);
+  static constexpr int LDS = N + N / 16;                     // padded LDS floats2 per line
+};
+
+// one padding slot every 16 elements breaks the stride-16 bank pattern of the first exchange
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+// Middle/last Stockham pass: NB butterflies of radix R per thread, span NS, inputs from LDS.
+template <int N, int R, int NS, int DIR, bool LAST>
+__device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2* lds, int t,
+                                              const float2* __restrict__ tw) {
+  constexpr int T = Line<N>::T;
+  constexpr int NB = N / (R * T);
+  constexpr int STRIDE = N / R;
+  float2 u[NB][R];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * T;
+    const int k = b % NS;
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[i][r] = lds[pad(b + r * STRIDE)];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[(r - 1) * NS + k];
+      if (DIR > 0) w.y = -w.y;
+      u[i][r] = cmul(u[i][r], w);
+    }
+    dft<R, DIR>(u[i]);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i + r * NB] = u[i][r];
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = t + i * T;
+      const int k = b % NS;
+      const int base = (b / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[pad(base + r * NS)] = u[i][r];
+    }
+    __syncthreads();
+  }
+}
+
+// Full line transform. On entry v[q] = x[t + q*T]; on exit v[q] = X[t + q*T].
+// Every thread of the workgroup must call this the same number of times (it contains barriers);
+// threads whose line is unused still call it on whatever data they hold.
+template <int N, int DIR>
+__device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds, int t,
+                                         const float2* __restrict__ tw) {
+  using L = Line<N>;
+  dft<L::R0, DIR>(v);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < L::R0; ++r) lds[pad(t * L::R0 + r)] = v[r];
+  __syncthreads();
+  if constexpr (L::NPASS == 2) {
+    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, tw + L::TW1);
+  } else {
+    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, tw + L::TW1);
+    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, tw + L::TW2);
+  }
+}
+
+}  // namespace s2t

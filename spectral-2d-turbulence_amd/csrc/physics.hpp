@@ -1,0 +1,213 @@
+// Shared device definitions: data layout in HBM, parameters, the integer hash and the pointwise
+// physics used both by the fused step kernels and by the test-hook kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "line_fft.hpp"
+
+namespace s2t {
+
+// ---------------------------------------------------------------------------------------------
+// Layout.
+//
+// The vorticity state is the spectrum of a real field, masked to the 2/3 band
+// (|kx|, |ky| <= K = floor(N/3), reference/spectral2d.py:22-23, common.wgsl:69-74). Only the
+// kx >= 0 half is stored (the kx < 0 half is its Hermitian mirror), transposed so that one
+// column kx is contiguous, and only the 2K+1 in-band rows:
+//   state[kx * PITCH + j], kx = 0..K, j = 0..2K; ky = j for j <= K, ky = j - (2K+1) otherwise.
+// Mixed-space fields (kx spectral, y physical) between the row and column kernels are stored for
+// kx = 0..K in 4-row tiles so that both a row and a group of columns read long runs:
+//   tile(y, kx, f) = ((y/4 * KC + kx) * F + f) * 4 + y%4  with F fields per tile.
+// ---------------------------------------------------------------------------------------------
+template <int N>
+struct Geo {
+  static constexpr int K = N / 3;             // N is never a multiple of 3
+  static constexpr int KC = K + 1;            // stored columns
+  static constexpr int NJ = 2 * K + 1;        // stored rows
+  static constexpr int PITCH = (NJ + 63) / 64 * 64;
+  static constexpr size_t STATE = (size_t)KC * PITCH;   // complex values per state field
+  static constexpr size_t PLANE = (size_t)N * KC;       // complex values per mixed field
+  // compact row of row index i in [0, N); -1 when out of band
+  static __device__ __forceinline__ int jrow(int i) {
+    return i <= K ? i : (i >= N - K ? i - (N - NJ) : -1);
+  }
+  template <int F>
+  static __device__ __forceinline__ size_t tile(int y, int kx, int f) {
+    return ((size_t)((y >> 2) * KC + kx) * F + f) * 4 + (y & 3);
+  }
+};
+
+__host__ __device__ __forceinline__ int wn(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// ---------------------------------------------------------------------------------------------
+// parameters (by value in kernel arguments)
+// ---------------------------------------------------------------------------------------------
+struct Phys {
+  double beta, mu, nu;   // nu = hyperviscosity coefficient nu_h
+  double p;              // dissipation order
+  int p_int;             // p when it is a small positive integer, else 0
+  float force_amp, force_k, force_width, force_norm;
+  double cfl, dt_min, dt_max, dx, fixed_dt;
+  float dye_fade;
+};
+
+struct Splat {
+  float x, y, dx, dy, r, g, b, radius, amp, dye_amp;
+};
+struct Splats {
+  int count;
+  Splat s[8];
+};
+
+// Per-solver device scalars. dt and dt-derived values never leave the device during stepping
+// (README.md:132-134 of the reference); the host reads them with the diagnostics.
+struct DevState {
+  double dt;            // step of the step in flight / last step
+  double t;             // simulated time
+  float speed;          // max |u|+|v| of the last step's initial field
+  unsigned speed_bits;  // running max for the step in flight (float bits, non-negative)
+  unsigned step;        // stepIndex of the step in flight (all ones before the first step)
+  unsigned nonfinite;   // sticky flag: a non-finite speed was seen
+};
+
+// ---------------------------------------------------------------------------------------------
+// integer hash (src/shaders/common.wgsl:76-84, src/gpu/solver.js:43-47)
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t hash1(uint32_t x) {
+  uint32_t v = x * 747796405u + 2891336453u;
+  v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
+  return (v >> 22u) ^ v;
+}
+__device__ __forceinline__ float hashf(uint32_t a, uint32_t b) {
+  return (float)hash1(a ^ hash1(b)) * 2.3283064365386963e-10f;
+}
+
+constexpr float TWO_PI_F = 6.28318530717958647f;
+
+// ---------------------------------------------------------------------------------------------
+// pointwise physics
+// ---------------------------------------------------------------------------------------------
+
+// u_hat = i ky w / k^2, v_hat = -i kx w / k^2 (solver.wgsl velocity_pack:21-40)
+__device__ __forceinline__ void vel_pack(float2 w, float kx, float ky, float inv, float2& uh,
+                                         float2& vh) {
+  const float2 iw = cmuli(w);
+  uh = cscale(iw, ky * inv);
+  vh = cscale(iw, -kx * inv);
+}
+
+// Hermitian unpack of a transform of two real fields: q1 = (z + conj zm)/2, q2 = -i(z - conj zm)/2
+__device__ __forceinline__ void herm_unpack(float2 z, float2 zm, float2& q1, float2& q2) {
+  q1 = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+  q2 = make_float2(0.5f * (z.y + zm.y), 0.5f * (zm.x - z.x));
+}
+
+// N_hat = i kx q1 + i ky q2 (solver.wgsl assemble:156)
+__device__ __forceinline__ float2 nhat(float2 q1, float2 q2, float kx, float ky) {
+  return make_float2(kx * (-q1.y) + ky * (-q2.y), kx * q1.x + ky * q2.x);
+}
+
+// Stochastic forcing of one in-band mode (solver.wgsl assemble:161-176). row, col are the full
+// grid indices of the mode; the canonical half is ky > 0, or ky = 0 and kx > 0.
+__device__ __forceinline__ float2 stoch_force(int row, int col, float kx, float ky, float k2,
+                                              const Phys& ph, uint32_t step, float dt, int n) {
+  const float x = (sqrtf(k2) - ph.force_k) / ph.force_width;
+  const float env = expf(-(x * x));
+  if (!(env > 1e-4f)) return make_float2(0.f, 0.f);
+  int hr = row, hc = col;
+  float sgn = 1.f;
+  if (!(ky > 0.f || (ky == 0.f && kx > 0.f))) {
+    hr = (n - row) % n;
+    hc = (n - col) % n;
+    sgn = -1.f;
+  }
+  const float phase = TWO_PI_F * hashf((uint32_t)(hr * n + hc), step);
+  const float amp = ph.force_amp * env * ph.force_norm / sqrtf(fmaxf(dt, 1e-6f));
+  float s, c;
+  sincosf(phase, &s, &c);
+  return make_float2(amp * c, sgn * amp * s);
+}
+
+// CFL step from the grid max of |u|+|v| (src/shaders/reduce.wgsl:82-88)
+__device__ __forceinline__ double cfl_dt(float speed, const Phys& ph) {
+  double dt = ph.dt_max;
+  if (speed > 1e-8f) dt = fmin(ph.dt_max, ph.cfl * ph.dx / (double)speed);
+  return fmax(dt, ph.dt_min);
+}
+
+// Integrating factor E(s) = exp(L s), L = -mu - nu (k^2)^p + i beta kx / k^2
+// (reference/spectral2d.py:26, solver.wgsl expL:97-101), evaluated in FP64 once per mode and
+// stage as E(dt/2); E(dt) = E(dt/2)^2 and E(-dt/2) = 1/E(dt/2).
+struct IFactor {
+  double mag, c, s;  // E(dt/2) = mag * (c + i s)
+};
+__device__ __forceinline__ IFactor ifactor_half(double kx, double k2, const Phys& ph, double dt) {
+  double k2p;
+  if (ph.p_int > 0) {
+    k2p = k2;
+    for (int i = 1; i < ph.p_int; ++i) k2p *= k2;
+  } else {
+    k2p = pow(k2, ph.p);
+  }
+  const double re = -ph.mu - ph.nu * k2p;
+  const double im = ph.beta * kx / k2;
+  IFactor e;
+  e.mag = exp(re * 0.5 * dt);
+  sincos(im * 0.5 * dt, &e.s, &e.c);
+  return e;
+}
+
+// RK coefficients [cA, a, cB, b] of the three IF-SSP-RK3 stages (src/gpu/solver.js:11-15):
+//   wout = cA E(a dt) w0 + cB E(b dt) (wcur + dt rhs)
+// evaluated in FP64 and rounded once.
+__device__ __forceinline__ float2 rk_update(int stage, const IFactor& e, float2 w0, float2 wcur,
+                                            float2 rhs, double dt) {
+  const double ax = (double)wcur.x + dt * (double)rhs.x;
+  const double ay = (double)wcur.y + dt * (double)rhs.y;
+  if (stage == 0) {
+    // E(dt) (w0 + dt f0)
+    const double m2 = e.mag * e.mag;
+    const double er = m2 * (e.c * e.c - e.s * e.s), ei = m2 * (2.0 * e.c * e.s);
+    return make_float2((float)(er * ax - ei * ay), (float)(er * ay + ei * ax));
+  }
+  if (stage == 1) {
+    // 3/4 E(dt/2) w0 + 1/4 E(-dt/2) (w1 + dt f1)
+    const double hr = e.mag * e.c, hi = e.mag * e.s;
+    const double ir = e.c / e.mag, ii = -e.s / e.mag;
+    const double x = 0.75 * (hr * w0.x - hi * w0.y) + 0.25 * (ir * ax - ii * ay);
+    const double y = 0.75 * (hr * w0.y + hi * w0.x) + 0.25 * (ir * ay + ii * ax);
+    return make_float2((float)x, (float)y);
+  }
+  // 1/3 E(dt) w0 + 2/3 E(dt/2) (w2 + dt f2)
+  const double m2 = e.mag * e.mag;
+  const double er = m2 * (e.c * e.c - e.s * e.s), ei = m2 * (2.0 * e.c * e.s);
+  const double hr = e.mag * e.c, hi = e.mag * e.s;
+  const double x = (1.0 / 3.0) * (er * w0.x - ei * w0.y) + (2.0 / 3.0) * (hr * ax - hi * ay);
+  const double y = (1.0 / 3.0) * (er * w0.y + ei * w0.x) + (2.0 / 3.0) * (hr * ay + hi * ax);
+  return make_float2((float)x, (float)y);
+}
+
+// periodic separation in unit coordinates (common.wgsl:42-48)
+__device__ __forceinline__ float sep(float a, float b) {
+  const float d = a - b + 0.5f;
+  return d - floorf(d) - 0.5f;
+}
+
+// curl of the Gaussian body forces at unit position (x, y) (src/shaders/force.wgsl:11-26)
+__device__ __forceinline__ float pointer_curl(float x, float y, const Splats& sp) {
+  float total = 0.f;
+  for (int i = 0; i < sp.count; ++i) {
+    const Splat s = sp.s[i];
+    const float rx = sep(x, s.x), ry = sep(y, s.y);
+    const float r2 = rx * rx + ry * ry;
+    const float sig2 = fmaxf(s.radius * s.radius, 1e-8f);
+    if (r2 < 9.f * sig2) {
+      const float g = expf(-r2 / sig2);
+      total = total + 2.f * s.amp * g * (s.dx * ry - s.dy * rx) / sig2;
+    }
+  }
+  return total;
+}
+
+}  // namespace s2t

@@ -1,0 +1,83 @@
+// Host-side solver object behind the C ABI, and the table of size-specialised launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/s2t.h"
+#include "physics.hpp"
+
+namespace s2t {
+
+struct Solver;
+
+// Launchers specialised on the grid size N (one table per supported N).
+struct Ops {
+  size_t state_elems;   // complex values per state field
+  size_t plane_elems;   // complex values per mixed-space field (N * (K+1))
+  int twiddle_elems;    // size of the line-FFT twiddle table
+  void (*twiddles)(float2* host);  // fill the FP64-derived twiddle table
+  // fused step kernels
+  void (*rows)(Solver* s, int stage, bool speed, bool vel, bool force);
+  void (*cols)(Solver* s, int stage, bool force);
+  void (*prime)(Solver* s);
+  // full-field helpers
+  void (*fft2)(Solver* s, float2* data, int inverse);
+  void (*pack)(Solver* s, const float2* full, float2* state);
+  void (*unpack)(Solver* s, const float2* state, float2* full);
+};
+
+const Ops* ops_for(int n);
+// fused right-hand side of the current mixed fields with explicit step index and dt (test hook)
+void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
+
+enum ColsMode { COLS_STEP = 0, COLS_PRIME = 1, COLS_RHS = 2 };
+
+struct Solver {
+  int n = 0, K = 0, dye_scale = 1, m = 0;
+  const Ops* ops = nullptr;
+  hipStream_t stream = nullptr;
+
+  // device buffers
+  float2* state[3] = {nullptr, nullptr, nullptr};  // compact half spectra, rotating
+  float2* mixed_u = nullptr;   // omega', u', v' tiles (F = 3)
+  float2* mixed_z = nullptr;   // P', Q' tiles (F = 2)
+  float2* mixed_f = nullptr;   // pointer-force F' tiles (F = 1), lazily
+  float2* force_k = nullptr;   // pointer-force spectrum, compact, lazily
+  float2* vel = nullptr;       // physical (u, v), N*N, lazily (dye)
+  float2* full = nullptr;      // full N*N complex scratch for seeding and hooks, lazily
+  float2* full2 = nullptr;     // second full scratch (hooks), lazily
+  float2* tw = nullptr;        // line-FFT twiddles
+  double2* wtab = nullptr;     // exp(2 pi i m / N), FP64, for the zonal mean
+  DevState* dev = nullptr;
+  double* diag_bins = nullptr; // [2][NBINS] energy and enstrophy per bin, then 2 totals
+  double* diag_zonal = nullptr;
+  double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation
+  ushort4* dye[3] = {nullptr, nullptr, nullptr};
+
+  // host mirror of configuration
+  Phys phys{};
+  s2t_params params{};
+  bool configured = false;
+  Splats splats{};
+  int phase = 0;        // state[phase] holds the current spectrum
+  int dye_parity = 0;
+  bool primed = false;  // mixed_u holds the prime of state[phase]
+
+  // profiling
+  bool prof = false;
+  struct Ev { hipEvent_t a, b; int kind; };
+  Ev* evs = nullptr;
+  int nev = 0, cap_ev = 0;
+  int prof_launches[S2T_PROF_KINDS] = {0, 0, 0, 0};
+  double prof_ms[S2T_PROF_KINDS] = {0, 0, 0, 0};
+};
+
+// profiling brackets around a launch
+void prof_begin(Solver* s, int kind);
+void prof_end(Solver* s);
+
+// launch-error check used by the launchers; records the error text and throws
+void check_launch(const char* what);
+
+}  // namespace s2t

@@ -1,0 +1,631 @@
+// Fused step kernels for gfx950, specialised on the grid size N.
+//
+// One right-hand-side evaluation plus its RK stage is two launches (the reference spends ten
+// dispatches on it, src/gpu/solver.js:496-517):
+//
+//   k_rows  (row kernel)   for a block of 4 rows y: inverse row FFTs of the packed half spectra
+//                          A' = omega' + i u' and B' = v'(y0) + i v'(y1) -> physical omega, u, v;
+//                          Z = u omega + i v omega (nl_pack, solver.wgsl:42-54); forward row FFT;
+//                          Hermitian unpack along x into P' = FFT_x(u omega), Q' = FFT_x(v omega)
+//                          for kx = 0..K. Stage 0 also reduces max |u|+|v| for the CFL step
+//                          (reduce.wgsl) and, when asked, writes the dye velocity and transforms
+//                          the pointer forcing (force.wgsl) alongside.
+//   k_cols  (column kernel) for C adjacent columns kx: forward column FFTs of P', Q' ->
+//                          N_hat = i kx P_hat + i ky Q_hat, rhs = (f_hat - N_hat) masked +
+//                          stochastic forcing (assemble, solver.wgsl:56-95); the IF-RK3 update
+//                          (rk3, solver.wgsl:103-122) written to the state; then the next
+//                          stage's velocity pack (velocity_pack, solver.wgsl:21-40) and inverse
+//                          column FFTs of omega_hat, u_hat, v_hat -> omega', u', v'.
+//
+// Both kernels stage each line in LDS only for the Stockham exchanges of line_fft; all global
+// accesses are coalesced runs (rows contiguous; columns through the 4-row tiles).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+
+#include "physics.hpp"
+#include "solver.hpp"
+
+namespace s2t {
+
+enum RowFlags { ROW_SPEED = 1, ROW_VEL = 2, ROW_FORCE = 4, ROW_CLOCK = 8 };
+
+struct RowArgs {
+  const float2* __restrict__ mu;  // omega', u', v' tiles (F = 3)
+  float2* __restrict__ mz;        // P', Q' tiles (F = 2)
+  float2* __restrict__ mf;        // F' tiles (F = 1)
+  float2* __restrict__ vel;       // physical (u, v), row-major N x N
+  const float2* __restrict__ tw;
+  DevState* dev;
+  Splats splats;
+  int stage;
+  int flags;
+};
+
+struct ColArgs {
+  const float2* __restrict__ mz;   // P', Q' tiles
+  const float2* __restrict__ mf;   // F' tiles (stage 0 with pointer forcing)
+  float2* __restrict__ mu;         // omega', u', v' tiles (out)
+  const float2* __restrict__ w0;   // state at the start of the step
+  const float2* __restrict__ wcur; // state entering this stage
+  float2* __restrict__ wout;       // state leaving this stage
+  float2* __restrict__ fk;         // pointer-force spectrum (compact layout)
+  float2* __restrict__ rhs_out;    // compact rhs (COLS_RHS)
+  const float2* __restrict__ tw;
+  DevState* dev;
+  Phys ph;
+  int stage;
+  int force;
+  uint32_t step_override;
+  double dt_override;
+};
+
+template <int N>
+struct RowCfg {
+  static constexpr int G = Line<N>::T >= 64 ? 1 : 64 / Line<N>::T;  // row blocks per workgroup
+};
+template <int N>
+struct ColCfg {
+  // columns per workgroup: 4 (2 for the longest lines, whose five line buffers per thread need
+  // more than the 128 VGPRs of a 1024-thread workgroup)
+  static constexpr int C = Line<N>::T >= 128 ? 2 : (Line<N>::T >= 16 ? 4 : 64 / Line<N>::T);
+  static constexpr int LSTRIDE = Line<N>::LDS + 2;                  // staggers column banks
+};
+
+// ---------------------------------------------------------------------------------------------
+// row kernel
+// ---------------------------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], float2* lds, int t,
+                                                 bool live, float2* __restrict__ out0,
+                                                 float2* __restrict__ out1, int y, int F) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < L::PTS; ++q) lds[pad(t + q * L::T)] = z[q];
+  __syncthreads();
+  if (!live) return;
+#pragma unroll
+  for (int q = 0; q < L::PTS; ++q) {
+    const int k = t + q * L::T;
+    if (k <= Ge::K) {
+      const float2 zk = lds[pad(k)];
+      const float2 zm = lds[pad((N - k) & (N - 1))];
+      float2 p, r;
+      herm_unpack(zk, zm, p, r);
+      if (F == 2) {
+        out0[Ge::template tile<2>(y, k, 0)] = p;
+        out0[Ge::template tile<2>(y, k, 1)] = r;
+      } else {
+        // a pair of real rows packed as re/im: p is row y, r is row y + 1
+        out0[Ge::template tile<1>(y, k, 0)] = p;
+        out1[Ge::template tile<1>(y + 1, k, 0)] = r;
+      }
+    }
+  }
+}
+
+template <int N, int G>
+__global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int T = L::T, PTS = L::PTS, K = Ge::K;
+  __shared__ float2 lds_all[G * L::LDS];
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int yb = blockIdx.x * G + g;
+  const bool live = yb < N / 4;
+  float2* lds = lds_all + g * L::LDS;
+
+  if ((a.flags & ROW_CLOCK) && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.stage == 0) {
+      a.dev->step += 1u;
+    } else if (a.stage == 1) {
+      a.dev->speed_bits = 0u;
+      a.dev->t += a.dev->dt;
+    }
+  }
+
+  float smax = 0.f;
+  bool bad = false;
+#pragma unroll 1
+  for (int pair = 0; pair < 2; ++pair) {
+    const int y0 = yb * 4 + pair * 2;
+    // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
+    float2 B[PTS];
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int k = t + q * T;
+      float2 b = make_float2(0.f, 0.f);
+      if (live) {
+        if (k == 0) {
+          const float2 v0 = a.mu[Ge::template tile<3>(y0, 0, 2)];
+          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, 0, 2)];
+          b = make_float2(v0.x, v1.x);
+        } else if (k <= K) {
+          const float2 v0 = a.mu[Ge::template tile<3>(y0, k, 2)];
+          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, k, 2)];
+          b = make_float2(v0.x - v1.y, v0.y + v1.x);
+        } else if (k >= N - K) {
+          const int m = N - k;
+          const float2 v0 = a.mu[Ge::template tile<3>(y0, m, 2)];
+          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, m, 2)];
+          b = make_float2(v0.x + v1.y, v1.x - v0.y);
+        }
+      }
+      B[q] = b;
+    }
+    line_fft<N, +1>(B, lds, t, a.tw);
+
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+      const int y = y0 + r;
+      float2 A[PTS];
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const int k = t + q * T;
+        float2 v = make_float2(0.f, 0.f);
+        if (live) {
+          if (k == 0) {
+            const float2 w = a.mu[Ge::template tile<3>(y, 0, 0)];
+            const float2 u = a.mu[Ge::template tile<3>(y, 0, 1)];
+            v = make_float2(w.x, u.x);
+          } else if (k <= K) {
+            const float2 w = a.mu[Ge::template tile<3>(y, k, 0)];
+            const float2 u = a.mu[Ge::template tile<3>(y, k, 1)];
+            v = make_float2(w.x - u.y, w.y + u.x);
+          } else if (k >= N - K) {
+            const int m = N - k;
+            const float2 w = a.mu[Ge::template tile<3>(y, m, 0)];
+            const float2 u = a.mu[Ge::template tile<3>(y, m, 1)];
+            v = make_float2(w.x + u.y, u.x - w.y);
+          }
+        }
+        A[q] = v;
+      }
+      line_fft<N, +1>(A, lds, t, a.tw);
+      // physical omega = Re A, u = Im A, v = Re/Im B; packed product Z = u omega + i v omega
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float om = A[q].x, u = A[q].y;
+        const float v = r == 0 ? B[q].x : B[q].y;
+        if (a.flags & ROW_SPEED) {
+          const float sp = fabsf(u) + fabsf(v);
+          bad |= !(sp <= 3.4e38f);
+          smax = fmaxf(smax, sp);
+        }
+        if ((a.flags & ROW_VEL) && live) a.vel[(size_t)y * N + t + q * T] = make_float2(u, v);
+        A[q] = make_float2(u * om, v * om);
+      }
+      line_fft<N, -1>(A, lds, t, a.tw);
+      row_unpack_store<N>(A, lds, t, live, a.mz, nullptr, y, 2);
+    }
+
+    if (a.flags & ROW_FORCE) {
+      // pointer forcing of rows y0, y0+1 at cell centres, packed as re/im of one transform
+      const float yu0 = ((float)y0 + 0.5f) / (float)N;
+      const float yu1 = ((float)(y0 + 1) + 0.5f) / (float)N;
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float xu = ((float)(t + q * T) + 0.5f) / (float)N;
+        B[q] = live ? make_float2(pointer_curl(xu, yu0, a.splats), pointer_curl(xu, yu1, a.splats))
+                    : make_float2(0.f, 0.f);
+      }
+      line_fft<N, -1>(B, lds, t, a.tw);
+      row_unpack_store<N>(B, lds, t, live, a.mf, a.mf, y0, 1);
+    }
+  }
+
+  if (a.flags & ROW_SPEED) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) smax = fmaxf(smax, __shfl_xor(smax, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&a.dev->speed_bits, __float_as_uint(smax));
+    if (bad) atomicOr(&a.dev->nonfinite, 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// column kernel
+// ---------------------------------------------------------------------------------------------
+template <int N, int C, int MODE, bool FORCE>
+__global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC;
+  constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
+  constexpr int GROUPS = (KC + C - 1) / C;
+  constexpr int G8 = (GROUPS + 7) / 8 * 8;
+  __shared__ float2 lds_all[C * LSTRIDE];
+
+  // XCD-aware mapping: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD
+  // a contiguous range of column groups (neighbouring tiles share L2 lines).
+  const int bid = blockIdx.x;
+  const int cg = (bid % 8) * (G8 / 8) + bid / 8;
+  if (cg >= GROUPS) return;
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int kx = cg * C + c;
+  const bool active = kx < KC;
+  float2* lds = lds_all + c * LSTRIDE;
+  const float fkx = (float)kx;
+  const size_t col = (size_t)kx * Ge::PITCH;
+
+  float2 W[PTS];
+  if constexpr (MODE == COLS_PRIME) {
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int j = Ge::jrow(t + q * T);
+      W[q] = (active && j >= 0) ? a.wcur[col + j] : make_float2(0.f, 0.f);
+    }
+  } else {
+    double dt;
+    uint32_t step;
+    if constexpr (MODE == COLS_RHS) {
+      dt = a.dt_override;
+      step = a.step_override;
+    } else {
+      step = a.dev->step;
+      if (a.stage == 0) {
+        const float speed = __uint_as_float(a.dev->speed_bits);
+        dt = a.ph.fixed_dt > 0.0 ? a.ph.fixed_dt : cfl_dt(speed, a.ph);
+        if (bid == 0 && tid == 0) {
+          a.dev->dt = dt;
+          a.dev->speed = speed;
+        }
+      } else {
+        dt = a.dev->dt;
+      }
+    }
+    float2 P[PTS], Q[PTS];
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int y = t + q * T;
+      P[q] = active ? a.mz[Ge::template tile<2>(y, kx, 0)] : make_float2(0.f, 0.f);
+      Q[q] = active ? a.mz[Ge::template tile<2>(y, kx, 1)] : make_float2(0.f, 0.f);
+    }
+    line_fft<N, -1>(P, lds, t, a.tw);
+    line_fft<N, -1>(Q, lds, t, a.tw);
+
+    float2 F[PTS];
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) F[q] = make_float2(0.f, 0.f);
+    if constexpr (FORCE) {
+      if (MODE == COLS_RHS || a.stage == 0) {
+#pragma unroll
+        for (int q = 0; q < PTS; ++q) {
+          const int y = t + q * T;
+          F[q] = active ? a.mf[Ge::template tile<1>(y, kx, 0)] : make_float2(0.f, 0.f);
+        }
+        line_fft<N, -1>(F, lds, t, a.tw);
+      } else {
+#pragma unroll
+        for (int q = 0; q < PTS; ++q) {
+          const int j = Ge::jrow(t + q * T);
+          F[q] = (active && j >= 0) ? a.fk[col + j] : make_float2(0.f, 0.f);
+        }
+      }
+    }
+
+    const float fdt = (float)dt;
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int idx = t + q * T;
+      const int j = Ge::jrow(idx);
+      float2 wv = make_float2(0.f, 0.f);
+      if (active && j >= 0) {
+        const float fky = (float)wn(idx, N);
+        const float k2 = fkx * fkx + fky * fky;
+        float2 rhs = make_float2(0.f, 0.f);
+        if (k2 > 0.f) {
+          const float2 nh = nhat(P[q], Q[q], fkx, fky);
+          rhs = FORCE ? csub(F[q], nh) : make_float2(-nh.x, -nh.y);
+          if (a.ph.force_amp > 0.f)
+            rhs = cadd(rhs, stoch_force(idx, kx, fkx, fky, k2, a.ph, step, fdt, N));
+          if constexpr (MODE == COLS_STEP) {
+            const float2 w0 = a.w0[col + j];
+            const float2 wc = a.stage == 0 ? w0 : a.wcur[col + j];
+            const IFactor e = ifactor_half((double)kx, (double)k2, a.ph, dt);
+            wv = rk_update(a.stage, e, w0, wc, rhs, dt);
+          }
+        }
+        if constexpr (MODE == COLS_RHS) {
+          a.rhs_out[col + j] = rhs;
+        } else {
+          a.wout[col + j] = wv;
+          if (FORCE && a.stage == 0) a.fk[col + j] = F[q];
+        }
+      }
+      W[q] = wv;
+    }
+    if constexpr (MODE == COLS_RHS) return;
+  }
+
+  // velocity pack of the new state and inverse column transforms; the 1/N per inverse pass of
+  // both directions (src/gpu/fft.js:59) is applied here as one exact 1/N^2
+  constexpr float S = 1.0f / ((float)N * (float)N);
+#pragma unroll 1
+  for (int f = 0; f < 3; ++f) {
+    float2 X[PTS];
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int idx = t + q * T;
+      const float fky = (float)wn(idx, N);
+      const float k2 = fkx * fkx + fky * fky;
+      const float inv = k2 > 0.f ? 1.f / k2 : 0.f;
+      const float2 w = cscale(W[q], S);
+      float2 uh, vh;
+      vel_pack(w, fkx, fky, inv, uh, vh);
+      X[q] = f == 0 ? w : (f == 1 ? uh : vh);
+    }
+    line_fft<N, +1>(X, lds, t, a.tw);
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) a.mu[Ge::template tile<3>(t + q * T, kx, f)] = X[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// full-field helpers: plain 2-D FFT (Fft2D.run, fft.js:81-92), state pack / unpack
+// ---------------------------------------------------------------------------------------------
+template <int N, int G, int DIR>
+__global__ void __launch_bounds__(G * Line<N>::T) k_fft_rows(float2* data, const float2* tw,
+                                                             float scale) {
+  using L = Line<N>;
+  __shared__ float2 lds_all[G * L::LDS];
+  const int g = threadIdx.x / L::T, t = threadIdx.x - g * L::T;
+  const int row = blockIdx.x * G + g;
+  const bool live = row < N;
+  float2* lds = lds_all + g * L::LDS;
+  float2 v[L::PTS];
+#pragma unroll
+  for (int q = 0; q < L::PTS; ++q)
+    v[q] = live ? data[(size_t)row * N + t + q * L::T] : make_float2(0.f, 0.f);
+  line_fft<N, DIR>(v, lds, t, tw);
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < L::PTS; ++q) data[(size_t)row * N + t + q * L::T] = cscale(v[q], scale);
+  }
+}
+
+template <int N, int C, int DIR>
+__global__ void __launch_bounds__(C * Line<N>::T) k_fft_cols(float2* data, const float2* tw,
+                                                             float scale) {
+  using L = Line<N>;
+  constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
+  __shared__ float2 lds_all[C * LSTRIDE];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int colx = blockIdx.x * C + c;
+  const bool live = colx < N;
+  float2* lds = lds_all + c * LSTRIDE;
+  float2 v[L::PTS];
+#pragma unroll
+  for (int q = 0; q < L::PTS; ++q)
+    v[q] = live ? data[(size_t)(t + q * L::T) * N + colx] : make_float2(0.f, 0.f);
+  line_fft<N, DIR>(v, lds, t, tw);
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < L::PTS; ++q) data[(size_t)(t + q * L::T) * N + colx] = cscale(v[q], scale);
+  }
+}
+
+// full N x N spectrum -> compact masked half spectrum, k = 0 zeroed (seed.wgsl:4-17)
+template <int N>
+__global__ void k_pack(const float2* __restrict__ full, float2* __restrict__ st) {
+  using Ge = Geo<N>;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Ge::STATE) return;
+  const int kx = (int)(i / Ge::PITCH), j = (int)(i % Ge::PITCH);
+  float2 v = make_float2(0.f, 0.f);
+  if (j < Ge::NJ && !(kx == 0 && j == 0)) {
+    const int row = j <= Ge::K ? j : j + (N - Ge::NJ);
+    v = full[(size_t)row * N + kx];
+  }
+  st[i] = v;
+}
+
+// compact half spectrum -> full N x N spectrum (Hermitian extension, zero out of band)
+template <int N>
+__global__ void k_unpack(const float2* __restrict__ st, float2* __restrict__ full) {
+  using Ge = Geo<N>;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * N) return;
+  const int row = (int)(i / N), colx = (int)(i % N);
+  float2 v = make_float2(0.f, 0.f);
+  if (colx <= Ge::K) {
+    const int j = Ge::jrow(row);
+    if (j >= 0) v = st[(size_t)colx * Ge::PITCH + j];
+  } else if (colx >= N - Ge::K) {
+    const int j = Ge::jrow((N - row) % N);
+    if (j >= 0) v = cconj(st[(size_t)(N - colx) * Ge::PITCH + j]);
+  }
+  full[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+template <int N>
+struct Impl {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  static constexpr int G = RowCfg<N>::G;
+  static constexpr int C = ColCfg<N>::C;
+
+  static void twiddles(float2* h) {
+    auto put = [&](int off, int ns, int r) {
+      for (int rr = 1; rr < r; ++rr)
+        for (int k = 0; k < ns; ++k) {
+          const long m = (long)rr * k;
+          const double ang = -2.0 * M_PI * (double)m / (double)(ns * r);
+          h[off + (rr - 1) * ns + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+    };
+    put(L::TW1, L::R0, L::R1);
+    if (L::NPASS == 3) put(L::TW2, L::R0 * L::R1, L::R2);
+  }
+
+  static void rows(Solver* s, int stage, bool speed, bool vel, bool force) {
+    RowArgs a{};
+    a.mu = s->mixed_u;
+    a.mz = s->mixed_z;
+    a.mf = s->mixed_f;
+    a.vel = s->vel;
+    a.tw = s->tw;
+    a.dev = s->dev;
+    a.splats = s->splats;
+    a.stage = stage;
+    a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0) |
+              (stage >= 0 ? ROW_CLOCK : 0);
+    if (stage < 0) a.stage = 1;
+    const int grid = (N / 4 + G - 1) / G;
+    prof_begin(s, S2T_PROF_ROWS);
+    hipLaunchKernelGGL((k_rows<N, G>), dim3(grid), dim3(G * L::T), 0, s->stream, a);
+    check_launch("k_rows");
+    prof_end(s);
+  }
+
+  static void cols(Solver* s, int stage, bool force) {
+    ColArgs a{};
+    a.mz = s->mixed_z;
+    a.mf = s->mixed_f;
+    a.mu = s->mixed_u;
+    const int p = s->phase;
+    // 3-buffer rotation of src/gpu/solver.js:193-212
+    a.w0 = s->state[p];
+    a.wcur = s->state[(p + stage) % 3];
+    a.wout = s->state[(p + (stage == 1 ? 2 : 1)) % 3];
+    a.fk = s->force_k;
+    a.tw = s->tw;
+    a.dev = s->dev;
+    a.ph = s->phys;
+    a.stage = stage;
+    a.force = force ? 1 : 0;
+    constexpr int GROUPS = (Ge::KC + C - 1) / C;
+    constexpr int G8 = (GROUPS + 7) / 8 * 8;
+    prof_begin(s, S2T_PROF_COLS);
+    if (force)
+      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, true>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+    check_launch("k_cols");
+    prof_end(s);
+  }
+
+  static void prime(Solver* s) {
+    ColArgs a{};
+    a.mu = s->mixed_u;
+    a.wcur = s->state[s->phase];
+    a.tw = s->tw;
+    a.dev = s->dev;
+    a.ph = s->phys;
+    constexpr int GROUPS = (Ge::KC + C - 1) / C;
+    constexpr int G8 = (GROUPS + 7) / 8 * 8;
+    prof_begin(s, S2T_PROF_OTHER);
+    hipLaunchKernelGGL((k_cols<N, C, COLS_PRIME, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+    check_launch("k_cols(prime)");
+    prof_end(s);
+  }
+
+  static void fft2(Solver* s, float2* data, int inverse) {
+    const float sc = inverse ? 1.0f / (float)N : 1.0f;
+    const int grow = (N + G - 1) / G;
+    const int gcol = (N + C - 1) / C;
+    if (inverse) {
+      hipLaunchKernelGGL((k_fft_rows<N, G, +1>), dim3(grow), dim3(G * L::T), 0, s->stream, data,
+                         s->tw, sc);
+      check_launch("k_fft_rows");
+      hipLaunchKernelGGL((k_fft_cols<N, C, +1>), dim3(gcol), dim3(C * L::T), 0, s->stream, data,
+                         s->tw, sc);
+      check_launch("k_fft_cols");
+    } else {
+      hipLaunchKernelGGL((k_fft_rows<N, G, -1>), dim3(grow), dim3(G * L::T), 0, s->stream, data,
+                         s->tw, sc);
+      check_launch("k_fft_rows");
+      hipLaunchKernelGGL((k_fft_cols<N, C, -1>), dim3(gcol), dim3(C * L::T), 0, s->stream, data,
+                         s->tw, sc);
+      check_launch("k_fft_cols");
+    }
+  }
+
+  static void pack(Solver* s, const float2* full, float2* st) {
+    const size_t n = Ge::STATE;
+    hipLaunchKernelGGL((k_pack<N>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream,
+                       full, st);
+    check_launch("k_pack");
+  }
+
+  static void unpack(Solver* s, const float2* st, float2* full) {
+    const size_t n = (size_t)N * N;
+    hipLaunchKernelGGL((k_unpack<N>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream,
+                       st, full);
+    check_launch("k_unpack");
+  }
+
+  static const Ops* ops() {
+    static const Ops o = {Ge::STATE, Ge::PLANE, L::TWSIZE, &twiddles, &rows, &cols, &prime,
+                          &fft2,     &pack,     &unpack};
+    return &o;
+  }
+};
+
+// RHS hook with explicit step index and dt (COLS_RHS reads them from the arguments)
+void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
+
+template <int N>
+static void cols_rhs_impl(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int C = ColCfg<N>::C;
+  ColArgs a{};
+  a.mz = s->mixed_z;
+  a.mf = s->mixed_f;
+  a.rhs_out = rhs_out;
+  a.tw = s->tw;
+  a.dev = s->dev;
+  a.ph = s->phys;
+  a.stage = 0;
+  a.force = force ? 1 : 0;
+  a.step_override = step;
+  a.dt_override = dt;
+  constexpr int GROUPS = (Ge::KC + C - 1) / C;
+  constexpr int G8 = (GROUPS + 7) / 8 * 8;
+  if (force)
+    hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, true>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+  else
+    hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+  check_launch("k_cols(rhs)");
+}
+
+void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
+  switch (s->n) {
+    case 16: return cols_rhs_impl<16>(s, force, rhs_out, step, dt);
+    case 32: return cols_rhs_impl<32>(s, force, rhs_out, step, dt);
+    case 64: return cols_rhs_impl<64>(s, force, rhs_out, step, dt);
+    case 128: return cols_rhs_impl<128>(s, force, rhs_out, step, dt);
+    case 256: return cols_rhs_impl<256>(s, force, rhs_out, step, dt);
+    case 512: return cols_rhs_impl<512>(s, force, rhs_out, step, dt);
+    case 1024: return cols_rhs_impl<1024>(s, force, rhs_out, step, dt);
+    case 2048: return cols_rhs_impl<2048>(s, force, rhs_out, step, dt);
+    case 4096: return cols_rhs_impl<4096>(s, force, rhs_out, step, dt);
+  }
+  throw std::runtime_error("unsupported grid size");
+}
+
+const Ops* ops_for(int n) {
+  switch (n) {
+    case 16: return Impl<16>::ops();
+    case 32: return Impl<32>::ops();
+    case 64: return Impl<64>::ops();
+    case 128: return Impl<128>::ops();
+    case 256: return Impl<256>::ops();
+    case 512: return Impl<512>::ops();
+    case 1024: return Impl<1024>::ops();
+    case 2048: return Impl<2048>::ops();
+    case 4096: return Impl<4096>::ops();
+  }
+  return nullptr;
+}
+
+}  // namespace s2t

@@ -1,0 +1,314 @@
+'use strict';
+// Node.js host interface of the MI355X solver: the reference's Solver / Fft2D / SCENARIOS surface
+// (src/gpu/solver.js, src/gpu/fft.js, src/scenarios.js) over the N-API addon s2t_napi.node,
+// which wraps the C ABI of include/s2t.h.
+//
+// Differences from the WebGPU original, by design:
+//  * there is no GPUDevice or command encoder: `device` arguments are accepted and ignored, and the
+//    encode* methods enqueue directly on the solver's HIP stream (Solver.step(n) runs n steps);
+//  * the velocity prime (encodePrime) and the CFL reduction (encodeReduce) are part of each step;
+//  * buffers live on the device; the test-visible internals of the browser harness are offered as
+//    explicit methods (readState / writeState, velocityPack, assemble, rhs, Fft2D on host arrays).
+
+const path = require('path');
+
+let native = null;
+function addon() {
+  if (!native) {
+    // throws when the addon has not been built: there is no JavaScript fallback
+    native = require(path.join(__dirname, 's2t_napi.node'));
+  }
+  return native;
+}
+
+const MAX_SPLATS = 8;
+
+// radixSchedule(n) of src/gpu/fft.js:20-31 (same result object and error text)
+function radixSchedule(n) {
+  return addon().radixSchedule(n);
+}
+
+// config defaults of src/main.js:9-36 for the keys the solver reads
+const DEFAULT_CONFIG = Object.freeze({
+  beta: 0,
+  drag: 0.01,
+  viscosity: 0.67,
+  hyperP: 4,
+  forceAmp: 0,
+  forceK: 22,
+  forceWidth: 2.5,
+  cfl: 0.5,
+  dtMin: 1e-5,
+  dtMax: 0.02,
+  dyeFade: 0.999,
+});
+
+class Solver {
+  constructor(handle, n, dyeScale) {
+    this.handle = handle;
+    this.n = n;
+    this.ncells = n * n;
+    this.m = n * dyeScale;
+    this.nbins = n / 2 + 1;
+    this.dyeScale = dyeScale;
+    this.simTime = 0;
+    this.disposed = false;
+    this.cfg = { ...DEFAULT_CONFIG };
+  }
+
+  // Solver.create(device, {n, dyeScale}) (solver.js:71-75)
+  static async create(device, { n = 256, dyeScale = 2 } = {}) {
+    const s = new Solver(addon().create(n, dyeScale), n, dyeScale);
+    s.configure(s.cfg);
+    return s;
+  }
+
+  _h() {
+    if (this.disposed) throw new Error('solver was destroyed');
+    return this.handle;
+  }
+
+  // configure(cfg) (solver.js:306-324); cfg.fixedDt > 0 replaces the CFL step
+  configure(cfg) {
+    this.cfg = { ...this.cfg, ...cfg };
+    addon().configure(this._h(), this.cfg);
+  }
+
+  // hyperviscosity coefficient in effect (paramsF[2] of the reference)
+  get nuH() {
+    return addon().nuH(this._h());
+  }
+
+  // setSplats(list) (solver.js:326-345), clamped to MAX_SPLATS
+  setSplats(list) {
+    addon().setSplats(this._h(), list.slice(0, MAX_SPLATS));
+  }
+
+  // seed({k0, amplitude, seed}) (solver.js:347-359)
+  seed({ k0 = 14, amplitude = 6, seed = 1 } = {}) {
+    addon().seedSpectrum(this._h(), k0, amplitude, seed);
+  }
+
+  // seedPhysical(values) (solver.js:361-382)
+  seedPhysical(values) {
+    const f = values instanceof Float32Array ? values : Float32Array.from(values);
+    addon().seedPhysical(this._h(), f, this.n);
+  }
+
+  // reset(opts) (solver.js:420-444)
+  reset(opts = {}) {
+    this.simTime = 0;
+    if (opts.field) {
+      this.seedPhysical(opts.field);
+    } else {
+      this.seed(opts);
+    }
+    this.paintDye(opts.dyePattern !== undefined ? opts.dyePattern : 'bands');
+  }
+
+  // paintDye(pattern) (solver.js:446-473)
+  paintDye(pattern) {
+    addon().paintDye(this._h(), pattern === 'none' ? 0 : 1);
+  }
+
+  // encodePrime: the prime is implicit in every step (kept for API parity)
+  encodePrime() {}
+
+  // encodeStep + encodeReduce (+ encodeDye when dye is true) for `count` steps
+  step(count = 1, { dye = false } = {}) {
+    addon().step(this._h(), count, !!dye);
+  }
+
+  encodeStep() {
+    this.step(1);
+  }
+
+  sync() {
+    addon().sync(this._h());
+  }
+
+  // readDiagnostics() (solver.js:570-608): FP64 record, or null after destroy()
+  async readDiagnostics() {
+    if (this.disposed) return null;
+    const d = await addon().diagnosticsAsync(this._h(), this.n);
+    this.simTime = d.time;
+    return d;
+  }
+
+  diagnostics() {
+    const d = addon().diagnostics(this._h(), this.n);
+    this.simTime = d.time;
+    return d;
+  }
+
+  get stepIndex() {
+    return addon().diagnostics(this._h(), this.n).stepIndex;
+  }
+
+  // current dye texture as binary16 rgba (Solver.currentDye)
+  get currentDye() {
+    return addon().readDye(this._h());
+  }
+
+  readState() {
+    return addon().readState(this._h(), this.n);
+  }
+
+  writeState(spectrum) {
+    const f = spectrum instanceof Float32Array ? spectrum : Float32Array.from(spectrum);
+    addon().writeState(this._h(), f, this.n);
+  }
+
+  // the pointwise stages on host arrays (reference harness, tests/index.html:161-182)
+  velocityPack(spectrum) {
+    const f = spectrum instanceof Float32Array ? spectrum : Float32Array.from(spectrum);
+    return addon().velocityPack(this._h(), f, this.n);
+  }
+
+  assemble(z, fhat = null, { stepIndex = 0, dt = 0.01 } = {}) {
+    const zf = z instanceof Float32Array ? z : Float32Array.from(z);
+    const ff = fhat === null ? null : fhat instanceof Float32Array ? fhat : Float32Array.from(fhat);
+    return addon().assemble(this._h(), zf, ff, this.n, stepIndex, dt);
+  }
+
+  rhs({ stepIndex = 0, dt = 0.01 } = {}) {
+    return addon().rhs(this._h(), this.n, stepIndex, dt);
+  }
+
+  destroy() {
+    this.disposed = true;
+    // the native handle is released by its finalizer when the last reference goes
+    this.handle = null;
+  }
+}
+
+// Fft2D (fft.js:33-101) on host arrays: forward unscaled, inverse scaled by 1/N per pass
+class Fft2D {
+  constructor(solver, n) {
+    this.solver = solver;
+    this.n = n;
+  }
+
+  static async create(device, n) {
+    radixSchedule(n);
+    return new Fft2D(await Solver.create(device, { n, dyeScale: 1 }), n);
+  }
+
+  run(data, inverse) {
+    const f = data instanceof Float32Array ? data : Float32Array.from(data);
+    return addon().fft2(this.solver._h(), f, this.n, !!inverse);
+  }
+
+  forward(data) {
+    return this.run(data, false);
+  }
+
+  inverse(data) {
+    return this.run(data, true);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// scenarios: physical vorticity fields on x_j = 2 pi j / N (src/scenarios.js:57-165)
+// ---------------------------------------------------------------------------------------------
+const TWO_PI = 2 * Math.PI;
+
+function onGrid(n, fn) {
+  const out = new Float32Array(n * n);
+  const h = TWO_PI / n;
+  for (let r = 0; r < n; r++) {
+    for (let c = 0; c < n; c++) out[r * n + c] = fn(c * h, r * h);
+  }
+  return out;
+}
+
+function periodicDelta(a, b) {
+  const d = a - b;
+  if (d > Math.PI) return d - TWO_PI;
+  if (d < -Math.PI) return d + TWO_PI;
+  return d;
+}
+
+// Gaussian vortices a^2-scaled, truncated at r^2 < 25 a^2
+function gaussians(n, cores) {
+  return onGrid(n, (x, y) => {
+    let sum = 0;
+    for (const v of cores) {
+      const dx = periodicDelta(x, v.x);
+      const dy = periodicDelta(y, v.y);
+      const r2 = dx * dx + dy * dy;
+      const a2 = v.a * v.a;
+      if (r2 < 25 * a2) sum += v.amp * Math.exp(-r2 / a2);
+    }
+    return sum;
+  });
+}
+
+const SCENARIOS = {
+  'shear layer': {
+    config: { substeps: 2, viscosity: 0.5, hyperP: 4, beta: 0, drag: 0, forceAmp: 0 },
+    build(n) {
+      const delta = Math.max(TWO_PI / 30, (6 * TWO_PI) / n);
+      return onGrid(n, (x, y) => {
+        const low = y <= Math.PI;
+        const z = (low ? y - Math.PI / 2 : 1.5 * Math.PI - y) / delta;
+        const c = Math.cosh(z);
+        return 0.05 * Math.cos(x) - (low ? 1 : -1) / (c * c) / delta;
+      });
+    },
+  },
+  'taylor-green': {
+    config: { substeps: 4, viscosity: 90, hyperP: 1, beta: 0, drag: 0, forceAmp: 0 },
+    build(n) {
+      return onGrid(n, (x, y) => 6 * Math.cos(2 * x) * Math.cos(2 * y));
+    },
+  },
+  dipole: {
+    config: { substeps: 4, viscosity: 0.3, hyperP: 4, beta: 0, drag: 0, forceAmp: 0 },
+    build(n) {
+      const a = 0.3;
+      const amp = 12 / (Math.PI * a * a);
+      const x = 0.6 * Math.PI;
+      return gaussians(n, [
+        { x, y: Math.PI - 0.45, a, amp },
+        { x, y: Math.PI + 0.45, a, amp: -amp },
+      ]);
+    },
+  },
+  merger: {
+    config: { substeps: 8, viscosity: 0.3, hyperP: 4, beta: 0, drag: 0, forceAmp: 0 },
+    build(n, cfg = {}) {
+      const d = cfg.mergerSep !== undefined ? cfg.mergerSep : 0.7;
+      return gaussians(n, [
+        { x: Math.PI - d / 2, y: Math.PI, a: 0.22, amp: 30 },
+        { x: Math.PI + d / 2, y: Math.PI, a: 0.22, amp: 30 },
+      ]);
+    },
+  },
+  polygon: {
+    config: { substeps: 8, viscosity: 0.22, hyperP: 4, beta: 0, drag: 0, forceAmp: 0 },
+    build(n, cfg = {}) {
+      const count = Math.round(cfg.polygonN !== undefined ? cfg.polygonN : 6);
+      const mode = Math.max(2, Math.floor(count / 2));
+      const cores = [];
+      for (let j = 0; j < count; j++) {
+        const th = (TWO_PI * j) / count;
+        const r = 1.25 * (1 + 0.02 * Math.cos((TWO_PI * mode * j) / count));
+        cores.push({ x: Math.PI + r * Math.cos(th), y: Math.PI + r * Math.sin(th), a: 0.19, amp: 34 });
+      }
+      return gaussians(n, cores);
+    },
+  },
+};
+
+const SCENARIO_NAMES = Object.keys(SCENARIOS);
+
+module.exports = {
+  Solver,
+  Fft2D,
+  SCENARIOS,
+  SCENARIO_NAMES,
+  MAX_SPLATS,
+  DEFAULT_CONFIG,
+  radixSchedule,
+};

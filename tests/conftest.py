@@ -1,0 +1,53 @@
+"""Shared test setup: the ``gpu`` marker, fixture loading and the ctypes binding."""
+
+import importlib.util
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: long CPU oracle runs (set S2T_SLOW=1)")
+
+
+def load_binding():
+    path = os.path.join(ROOT, "spectral-2d-turbulence_amd", "__init__.py")
+    spec = importlib.util.spec_from_file_location("s2t_amd", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def s2t():
+    return load_binding()
+
+
+@pytest.fixture(scope="session")
+def fixtures():
+    with open(os.path.join(GOLDEN, "fixtures.json")) as fh:
+        return json.load(fh)
+
+
+def cplx(flat, n):
+    """Fixture layout (interleaved re, im, row-major; make_fixtures.py:11-16) -> complex N x N."""
+    a = np.asarray(flat, np.float64)
+    return (a[0::2] + 1j * a[1::2]).reshape(n, n)
+
+
+def rel_err(got, want):
+    """max|got - want| / max|want| over the interleaved floats (tests/index.html:89-97)."""
+    g = np.concatenate([np.real(got).ravel(), np.imag(got).ravel()])
+    w = np.concatenate([np.real(want).ravel(), np.imag(want).ravel()])
+    den = np.max(np.abs(w))
+    num = np.max(np.abs(g - w))
+    return num / den if den > 0 else num

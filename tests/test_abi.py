@@ -1,0 +1,49 @@
+"""The C ABI library loads and exports every entry point of include/s2t.h (no GPU calls)."""
+
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "s2t.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(s2t_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding(s2t):
+    assert header_symbols() == sorted(s2t.EXPORTS)
+
+
+def test_library_exports(s2t):
+    lib = s2t.load()
+    for name in header_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_radix_schedule_and_bad_sizes(s2t):
+    import ctypes
+    lib = s2t.load()
+    st, r2 = ctypes.c_int(), ctypes.c_int()
+    assert lib.s2t_radix_schedule(128, ctypes.byref(st), ctypes.byref(r2)) == 0
+    assert (st.value, r2.value) == (3, 1)
+    assert lib.s2t_radix_schedule(4096, ctypes.byref(st), ctypes.byref(r2)) == 0
+    assert (st.value, r2.value) == (6, 0)
+    assert lib.s2t_radix_schedule(96, None, None) == s2t.S2T_EINVAL
+    # message of src/gpu/fft.js:28
+    assert lib.s2t_last_error().decode() == "grid size 96 must be 4^a or 2 * 4^a"
+    h = ctypes.c_void_p()
+    assert lib.s2t_create(100, 1, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert "must be 4^a or 2 * 4^a" in lib.s2t_last_error().decode()
+    assert lib.s2t_create(8192, 1, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert lib.s2t_create(256, 3, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert not h.value
+
+
+def test_null_handles_rejected(s2t):
+    lib = s2t.load()
+    assert lib.s2t_step(None, 1, 0) == s2t.S2T_EINVAL
+    assert lib.s2t_destroy(None) == 0

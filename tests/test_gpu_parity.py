@@ -1,0 +1,386 @@
+"""HIP kernels through the C ABI against the FP64 oracle and the reference's golden vectors.
+
+Tolerances are written per test. The reference harness bounds (tests/index.html:136-201) are
+2e-5 (fft, velocity pack), 5e-5 (assemble, one step) and 5e-3 (20 steps); the bounds here are
+tighter, and the one-step bound 1e-5 is the accuracy target for FP32 steps against FP64.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import cplx, rel_err
+from oracle import gpusem as gs
+from oracle.spectral import TWO_PI, Grid, Scheme, grid_coords, nu_from_rate, random_field
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+def fixture_params(fx):
+    st = fx["step"]
+    # tests/index.html:122-134
+    return dict(beta=st["beta"], drag=st["mu"], viscosity=1 / 1.5, hyper_p=st["hyperP"],
+                force_amp=0.0, force_k=20.0, force_width=2.0, cfl=0.5, dt_min=1e-6, dt_max=1.0,
+                dye_fade=1.0, fixed_dt=st["dt"])
+
+
+def oracle_for(params, n):
+    return Scheme(n, beta=params["beta"], mu=params["drag"],
+                  nu=nu_from_rate(params["viscosity"], n, params["hyper_p"]),
+                  p=int(params["hyper_p"]), cfl=params["cfl"], dt_max=params["dt_max"])
+
+
+@pytest.fixture
+def solver32(s2t, fixtures):
+    s = s2t.Solver(32)
+    s.configure(**fixture_params(fixtures))
+    yield s
+    s.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# golden vectors (n = 32)
+# ---------------------------------------------------------------------------------------------
+def test_fft_fixture(solver32, fixtures):
+    f = fixtures["fft"]
+    x = cplx(f["input"], 32)
+    assert rel_err(solver32.fft2(x), cplx(f["forward"], 32)) < 1e-6
+    assert rel_err(solver32.fft2(x, inverse=True), cplx(f["inverse"], 32)) < 1e-6
+
+
+def test_nu_h_fixture(solver32, fixtures):
+    assert abs(solver32.nu_h() / fixtures["step"]["nuH"] - 1) < 1e-12
+
+
+def test_velocity_pack_fixture(solver32, fixtures):
+    v = fixtures["velocity"]
+    a, b = solver32.velocity_pack(cplx(v["what"], 32))
+    assert rel_err(a, cplx(v["packA"], 32)) < 1e-6
+    assert rel_err(b, cplx(v["packB"], 32)) < 1e-6
+
+
+def test_assemble_fixture(solver32, fixtures):
+    a = fixtures["assemble"]
+    got = solver32.assemble(cplx(a["z"], 32))
+    assert rel_err(got, cplx(a["rhs"], 32)) < 1e-6
+
+
+def test_fused_rhs_fixture(solver32, fixtures):
+    """The fused row+column kernels reproduce -N_hat of the assemble fixture."""
+    solver32.write_state(cplx(fixtures["velocity"]["what"], 32))
+    got = solver32.rhs()
+    assert rel_err(got, cplx(fixtures["assemble"]["rhs"], 32)) < 2e-6
+
+
+def test_state_roundtrip(solver32, fixtures):
+    w = cplx(fixtures["step"]["before"], 32)
+    solver32.write_state(w)
+    assert rel_err(solver32.read_state(), w) < 1e-7
+
+
+def test_step_fixture(solver32, fixtures):
+    st = fixtures["step"]
+    solver32.write_state(cplx(st["before"], 32))
+    solver32.step(1)
+    assert rel_err(solver32.read_state(), cplx(st["after"], 32)) < 1e-5
+
+
+def test_run20_fixture(solver32, fixtures):
+    st, r = fixtures["step"], fixtures["run20"]
+    solver32.write_state(cplx(st["before"], 32))
+    solver32.step(20)
+    assert rel_err(solver32.read_state(), cplx(r["after"], 32)) < 1e-4
+    d = solver32.diagnostics()
+    assert d["energy"] == pytest.approx(r["energy"], rel=1e-5)
+    assert d["enstrophy"] == pytest.approx(r["enstrophy"], rel=1e-5)
+    assert d["stepIndex"] == 20
+    assert d["time"] == pytest.approx(20 * st["dt"], rel=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# FFT at every supported size
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", SIZES)
+def test_fft_sizes(s2t, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    with s2t.Solver(n) as s:
+        fwd = s.fft2(x)
+        inv = s.fft2(x, inverse=True)
+    tol = 2e-7 * np.log2(n) + 1e-7
+    assert rel_err(fwd, np.fft.fft2(x)) < tol
+    assert rel_err(inv, np.fft.ifft2(x)) < tol
+
+
+# ---------------------------------------------------------------------------------------------
+# one step against the FP64 oracle at every size (fixed and CFL dt)
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [16, 64, 128, 256, 512, 1024, 2048])
+def test_one_step_vs_oracle(s2t, n):
+    params = dict(beta=10.0, drag=0.01, viscosity=1.0, hyper_p=4.0, force_amp=0.0, force_k=8.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    o = oracle_for(params, n)
+    o.set_physical(random_field(n, k0=max(2.0, n / 16), seed=n, amplitude=6.0))
+    u, v = o.velocity()
+    dt = gs.gpu_dt(float(np.max(np.abs(u) + np.abs(v))), 0.5, TWO_PI / n, 1e-5, 0.02)
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.write_state(o.w)
+        s.step(1)
+        got = s.read_state()
+        d = s.diagnostics()
+    assert d["dt"] == pytest.approx(dt, rel=1e-5)
+    o.step(dt=d["dt"])
+    assert rel_err(got, o.w) < 1e-5
+
+
+def test_one_step_4096_vs_oracle(s2t):
+    """The accuracy statement at the bench size: one FP32 step vs FP64, relative error <= 1e-5."""
+    n = 4096
+    params = dict(beta=10.0, drag=0.01, viscosity=1.0, hyper_p=4.0, force_amp=0.0, force_k=32.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    o = oracle_for(params, n)
+    o.w = gs.fill_spectrum(n, k0=64.0, amplitude=6.0, seed=7).astype(np.complex128)
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.seed_spectrum(k0=64.0, amplitude=6.0, seed=7)
+        w0 = s.read_state()
+        assert rel_err(w0, o.w) < 1e-6
+        s.step(1)
+        got = s.read_state()
+        dt = s.diagnostics()["dt"]
+    o.step(dt=dt)
+    err = rel_err(got, o.w)
+    print("4096 one-step relative error vs FP64: %.3e" % err)
+    assert err < 1e-5
+
+
+def test_twenty_steps_cfl_vs_oracle(s2t):
+    n = 256
+    params = dict(beta=5.0, drag=0.0, viscosity=0.5, hyper_p=4.0, force_amp=0.0, force_k=8.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    o = oracle_for(params, n)
+    o.set_physical(random_field(n, k0=12, seed=3, amplitude=6.0))
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.write_state(o.w)
+        for _ in range(20):
+            s.step(1)
+            o.step(dt=s.diagnostics()["dt"])
+        got = s.read_state()
+    assert rel_err(got, o.w) < 1e-4
+
+
+# ---------------------------------------------------------------------------------------------
+# closed forms (Taylor-Green, config 1: 128^2, nu = 1e-3 Laplacian, dt = 1e-3, 1000 steps)
+# ---------------------------------------------------------------------------------------------
+def test_taylor_green_c1(s2t):
+    n, nu = 128, 1e-3
+    rate = nu * (n / 3.0) ** 2  # viscosity is the rate at kcut
+    gx, gy = grid_coords(n)
+    field = 6.0 * np.cos(2 * gx) * np.cos(2 * gy)
+    with s2t.Solver(n) as s:
+        s.configure(beta=0.0, drag=0.0, viscosity=rate, hyper_p=1.0, force_amp=0.0,
+                    fixed_dt=1e-3)
+        s.seed_physical(field)
+        w0 = s.read_state()
+        nl = s.rhs()
+        assert np.max(np.abs(nl)) / np.max(np.abs(w0)) < 1e-6
+        s.step(1000)
+        w1 = s.read_state()
+    ratio = w1[2, 2] / w0[2, 2]
+    expect = np.exp(-8.0 * nu * 1.0)
+    assert abs(abs(ratio) / expect - 1) < 1e-6
+    # shape preserved: only the four (+-2, +-2) modes carry energy
+    mask = np.zeros((n, n), bool)
+    for r in (2, n - 2):
+        for c in (2, n - 2):
+            mask[r, c] = True
+    assert np.max(np.abs(w1[~mask])) < 1e-6 * np.max(np.abs(w1))
+
+
+def test_inviscid_invariants(s2t):
+    n = 256
+    with s2t.Solver(n) as s:
+        s.configure(beta=12.0, drag=0.0, viscosity=0.0, hyper_p=1.0, force_amp=0.0,
+                    fixed_dt=2e-3)
+        s.seed_physical(random_field(n, k0=10, seed=2))
+        d0 = s.diagnostics()
+        s.step(200)
+        d1 = s.diagnostics()
+    assert abs(d1["energy"] / d0["energy"] - 1) < 1e-5
+    assert abs(d1["enstrophy"] / d0["enstrophy"] - 1) < 1e-5
+
+
+# ---------------------------------------------------------------------------------------------
+# seeding, scenarios and diagnostics
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [32, 256, 1024])
+def test_seed_spectrum(s2t, n):
+    want = gs.fill_spectrum(n, k0=14.0, amplitude=6.0, seed=1)
+    with s2t.Solver(n) as s:
+        s.seed_spectrum(k0=14.0, amplitude=6.0, seed=1)
+        got = s.read_state()
+        d = s.diagnostics()
+    assert rel_err(got, want) < 1e-6
+    assert d["stepIndex"] == 0 and d["time"] == 0.0
+
+
+@pytest.mark.parametrize("name", ["taylor-green", "dipole", "merger", "polygon", "shear layer"])
+def test_seed_physical_scenarios(s2t, name):
+    n = 256
+    field = gs.scenario_field(name, n, {"mergerSep": 0.9, "polygonN": 6})
+    o = Scheme(n)
+    o.set_physical(field)
+    with s2t.Solver(n) as s:
+        s.seed_physical(field.astype(np.float32))
+        got = s.read_state()
+    assert rel_err(got, o.w) < 1e-6
+
+
+def test_diagnostics_vs_oracle(s2t):
+    n = 512
+    o = Scheme(n)
+    o.set_physical(random_field(n, k0=20, seed=9, amplitude=4.0))
+    with s2t.Solver(n) as s:
+        s.write_state(o.w)
+        w = s.read_state()
+        d = s.diagnostics()
+    assert d["energy"] == pytest.approx(o.energy(w), rel=1e-12)
+    assert d["enstrophy"] == pytest.approx(o.enstrophy(w), rel=1e-12)
+    spec = o.spectrum(w)
+    assert np.max(np.abs(d["spectrum"] - spec)) < 1e-12 * np.max(spec)
+    zon = o.zonal_mean_u(w)
+    assert np.max(np.abs(d["zonal"] - zon)) < 1e-12 * max(1.0, np.max(np.abs(zon)))
+
+
+def test_cfl_and_clock(s2t):
+    n = 128
+    with s2t.Solver(n) as s:
+        s.configure(cfl=0.5, dt_min=1e-5, dt_max=0.02, fixed_dt=0.0)
+        s.seed_spectrum(k0=0.001, amplitude=0.0, seed=1)  # at rest
+        s.step(3)
+        d = s.diagnostics()
+        assert d["dt"] == 0.02 and d["maxSpeed"] == 0.0 and d["stepIndex"] == 3
+        assert d["time"] == pytest.approx(0.06)
+        s.seed_spectrum(k0=10.0, amplitude=50.0, seed=2)
+        w = s.read_state()
+        s.step(1)
+        d = s.diagnostics()
+    o = Scheme(n)
+    u, v = o.velocity(w)
+    speed = float(np.max(np.abs(u) + np.abs(v)))
+    assert d["maxSpeed"] == pytest.approx(speed, rel=1e-5)
+    assert d["dt"] == pytest.approx(gs.gpu_dt(speed, 0.5, TWO_PI / n, 1e-5, 0.02), rel=1e-5)
+
+
+def test_nonfinite_flag(s2t):
+    n = 64
+    w = np.zeros((n, n), np.complex128)
+    w[1, 2] = np.nan
+    with s2t.Solver(n) as s:
+        s.write_state(w)
+        s.step(1)
+        assert s.diagnostics()["nonfinite"] == 1
+
+
+# ---------------------------------------------------------------------------------------------
+# forcing
+# ---------------------------------------------------------------------------------------------
+def test_stochastic_forcing_assemble_hook(s2t):
+    n = 128
+    with s2t.Solver(n) as s:
+        s.configure(force_amp=5.0, force_k=20.0, force_width=2.5)
+        got = s.assemble(np.zeros((n, n), np.complex64), step_index=41, dt=0.004)
+    want = gs.stochastic_forcing(n, 5.0, 20.0, 2.5, 41, 0.004)
+    assert rel_err(got, want) < 2e-5
+
+
+def test_stochastic_forcing_step_vs_oracle(s2t):
+    """Forced steps from rest: the forcing phases follow the device step index."""
+    n = 128
+    params = dict(beta=10.0, drag=0.01, viscosity=1.0, hyper_p=4.0, force_amp=5.0, force_k=20.0,
+                  force_width=2.5, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0,
+                  fixed_dt=0.005)
+    o = oracle_for(params, n)
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.seed_spectrum(k0=1.0, amplitude=0.0, seed=1)
+        s.step(5)
+        got = s.read_state()
+    for k in range(5):
+        o.step(dt=0.005, forcing=gs.stochastic_forcing(n, 5.0, 20.0, 2.5, k, 0.005))
+    assert rel_err(got, o.w) < 2e-5
+
+
+def test_pointer_forcing_rhs_and_step(s2t):
+    n = 256
+    splats = [dict(x=0.3, y=0.6, dx=0.02, dy=-0.01, radius=0.05, amp=400.0, color=(1, 0, 0),
+                   dyeAmp=0.5),
+              dict(x=0.95, y=0.02, dx=-0.01, dy=0.03, radius=0.03, amp=250.0, color=(0, 1, 0),
+                   dyeAmp=0.2)]
+    o = Scheme(n, beta=0.0, mu=0.0, nu=0.0)
+    o.set_physical(random_field(n, k0=12, seed=4, amplitude=2.0))
+    fh = np.where(o.g.band, gs.pointer_force_spectrum(n, splats), 0.0)
+    with s2t.Solver(n) as s:
+        s.configure(beta=0.0, drag=0.0, viscosity=0.0, hyper_p=1.0, force_amp=0.0, fixed_dt=0.004)
+        s.set_splats(splats)
+        s.write_state(o.w)
+        got_rhs = s.rhs()
+        s.step(3)
+        got = s.read_state()
+    want_rhs = o.rhs(o.w, fh)
+    want_rhs[0, 0] = 0.0
+    assert rel_err(got_rhs, want_rhs) < 2e-5
+    for _ in range(3):
+        o.step(dt=0.004, forcing=fh)
+    assert rel_err(got, o.w) < 2e-5
+
+
+# ---------------------------------------------------------------------------------------------
+# dye
+# ---------------------------------------------------------------------------------------------
+def test_paint_bands(s2t):
+    with s2t.Solver(64, dye_scale=2) as s:
+        s.paint_dye("bands")
+        got = s.read_dye()
+    want = gs.paint_bands(128)
+    assert np.array_equal(got.view(np.uint16), want.view(np.uint16))
+
+
+def test_dye_step_vs_oracle(s2t):
+    n, m = 64, 128
+    rng = np.random.default_rng(3)
+    dye = np.ones((m, m, 4), np.float16)
+    dye[..., :3] = rng.uniform(0, 1, (m, m, 3)).astype(np.float16)
+    o = Scheme(n)
+    o.set_physical(random_field(n, k0=6, seed=8, amplitude=6.0))
+    u, v = o.velocity()
+    vel = np.stack([u, v], axis=-1).astype(np.float32)
+    splats = [dict(x=0.5, y=0.5, dx=0.01, dy=0.0, radius=0.08, amp=0.0, color=(0.2, 0.4, 0.6),
+                   dyeAmp=0.7)]
+    with s2t.Solver(n, dye_scale=2) as s:
+        s.configure(dye_fade=0.99)
+        s.set_splats(splats)
+        s.write_dye(dye)
+        s.write_velocity(vel)
+        s.dye_step(0.02)
+        got = s.read_dye().astype(np.float64)
+    want = gs.dye_step(dye, vel, 0.02, fade=0.99, splats=splats).astype(np.float64)
+    diff = np.abs(got - want)
+    assert np.quantile(diff, 0.999) < 2e-3
+    assert diff.max() < 3e-2
+
+
+def test_step_with_dye_moves_dye(s2t):
+    n = 128
+    with s2t.Solver(n, dye_scale=2) as s:
+        s.seed_spectrum(k0=8.0, amplitude=6.0, seed=3)
+        s.paint_dye("bands")
+        d0 = s.read_dye().astype(np.float64)
+        s.step(5, dye=True)
+        d1 = s.read_dye().astype(np.float64)
+    assert np.isfinite(d1).all()
+    assert np.abs(d1 - d0).max() > 1e-3
+    assert d1[..., :3].max() <= d0[..., :3].max() + 1e-3 and d1[..., 3].min() == 1.0

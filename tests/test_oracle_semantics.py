@@ -1,0 +1,162 @@
+"""Pins for the oracle's restatement of GPU-only semantics (oracle/gpusem.py).
+
+None of these functions has an FP64 counterpart in the reference; they are pinned by known
+answers from independent integer arithmetic and by what the mathematics fixes.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import gpusem as gs
+from oracle.spectral import TWO_PI, Grid, Scheme
+
+# hash1 known answers computed with C uint32 arithmetic (gcc), independent of numpy's wrap-around
+HASH_KAT = [(0, 129708002), (1, 2831084092), (2, 2055130248), (7, 2120684060),
+            (1000, 2163870295), (123456789, 4272394698), (4294967295, 3861530882),
+            (2891336453, 582399676)]
+
+
+def test_hash_known_answers():
+    for x, h in HASH_KAT:
+        assert int(gs.hash1(np.uint32(x))) == h
+    assert int(gs.hash1(np.uint32(5) ^ gs.hash1(np.uint32(3)))) == 1861624210
+
+
+def test_hashf_uniform():
+    v = gs.hashf(np.arange(200000, dtype=np.uint32), np.uint32(17)).astype(np.float64)
+    assert v.min() >= 0 and v.max() <= 1.0
+    hist, _ = np.histogram(v, bins=20, range=(0, 1))
+    assert np.all(np.abs(hist / 10000.0 - 1) < 0.05)
+
+
+def test_gpu_dt():
+    assert gs.gpu_dt(0.0, 0.5, 0.1, 1e-5, 0.02) == 0.02
+    assert gs.gpu_dt(float("nan"), 0.5, 0.1, 1e-5, 0.02) == 0.02
+    assert gs.gpu_dt(100.0, 0.5, 0.1, 1e-5, 0.02) == pytest.approx(5e-4)
+    assert gs.gpu_dt(1e9, 0.5, 0.1, 1e-5, 0.02) == 1e-5
+
+
+def test_stochastic_forcing_is_hermitian_and_in_annulus():
+    n, kf, width = 64, 10.0, 2.0
+    f = gs.stochastic_forcing(n, 3.0, kf, width, 12, 0.01)
+    g = Grid(n)
+    m = g.mirror
+    assert np.max(np.abs(f - np.conj(f[np.ix_(m, m)]))) < 1e-9 * np.max(np.abs(f))
+    kr = np.sqrt(g.k2)
+    nz = np.abs(f) > 0
+    assert nz.sum() > 0
+    assert np.all(np.abs(kr[nz] - kf) < width * np.sqrt(np.log(1e4)) + 1e-9)
+    assert np.all(g.band[nz])
+    # real physical field
+    assert np.max(np.abs(np.fft.ifft2(f).imag)) < 1e-9 * np.max(np.abs(np.fft.ifft2(f).real))
+
+
+def test_stochastic_forcing_scaling():
+    n = 64
+    f1 = gs.stochastic_forcing(n, 2.0, 10.0, 2.0, 5, 0.01)
+    f2 = gs.stochastic_forcing(n, 2.0, 10.0, 2.0, 5, 0.04)
+    f3 = gs.stochastic_forcing(n, 4.0, 10.0, 2.0, 5, 0.01)
+    assert np.allclose(f2, f1 / 2.0)
+    assert np.allclose(f3, 2.0 * f1)
+    f4 = gs.stochastic_forcing(n, 2.0, 10.0, 2.0, 6, 0.01)
+    assert np.allclose(np.abs(f4), np.abs(f1)) and not np.allclose(f4, f1)
+
+
+def test_injection_rate_closed_form():
+    """Energy injected from rest by forcing alone matches 1/2 (0.01 A)^2 sum env^2/k^2."""
+    n, amp, kf, width, dt = 64, 3.0, 10.0, 2.0, 0.01
+    s = Scheme(n)
+    g = s.g
+    e = []
+    for step in range(400):
+        w = np.zeros((n, n), np.complex128)
+        w += dt * gs.stochastic_forcing(n, amp, kf, width, step, dt)
+        e.append(0.5 * np.sum(np.abs(w) ** 2 * g.inv_k2) / n ** 4)
+    rate = np.mean(e) / dt
+    assert rate == pytest.approx(gs.injection_rate(n, amp, kf, width), rel=0.1)
+    assert gs.amp_for_rate(gs.injection_rate(n, amp, kf, width), n, kf, width) == pytest.approx(amp)
+
+
+def test_pointer_force_is_curl_of_gaussian_force():
+    """Closed-form curl vs a centred finite difference of the body force on a fine grid."""
+    n = 512
+    sp = dict(x=0.4, y=0.55, dx=0.03, dy=-0.02, radius=0.05, amp=7.0)
+    got = gs.pointer_force_physical(n, [sp])
+    c = (np.arange(n) + 0.5) / n
+    y, x = np.meshgrid(c, c, indexing="ij")
+    rx = gs._sep(x, sp["x"])
+    ry = gs._sep(y, sp["y"])
+    g = np.exp(-(rx ** 2 + ry ** 2) / sp["radius"] ** 2)
+    fx, fy = sp["amp"] * g * sp["dx"], sp["amp"] * g * sp["dy"]
+    h = 1.0 / n
+    curl = (np.roll(fy, -1, 1) - np.roll(fy, 1, 1)) / (2 * h) - (np.roll(fx, -1, 0) - np.roll(fx, 1, 0)) / (2 * h)
+    inside = rx ** 2 + ry ** 2 < 4 * sp["radius"] ** 2
+    assert np.max(np.abs(got[inside] - curl[inside])) / np.max(np.abs(curl)) < 5e-3
+    flipped = gs.pointer_force_physical(n, [dict(sp, dx=-sp["dx"], dy=-sp["dy"])])
+    assert np.allclose(flipped, -got)
+    assert abs(got.mean()) < 1e-6 * np.max(np.abs(got))
+
+
+def test_fill_spectrum():
+    n = 64
+    w = gs.fill_spectrum(n, k0=8.0, amplitude=6.0, seed=3).astype(np.complex128)
+    g = Grid(n)
+    m = g.mirror
+    assert np.max(np.abs(w - np.conj(w[np.ix_(m, m)]))) < 1e-5 * np.max(np.abs(w))
+    assert np.all(w[~g.band] == 0) and w[0, 0] == 0
+    omega = np.fft.ifft2(w).real
+    assert np.sqrt(np.mean(omega ** 2)) == pytest.approx(6.0, rel=1e-5)
+    w2 = gs.fill_spectrum(n, k0=8.0, amplitude=6.0, seed=4)
+    assert np.allclose(np.abs(w2), np.abs(w), rtol=1e-5) and not np.allclose(w2, w)
+
+
+def test_scenarios_match_formulas():
+    n = 64
+    x = np.arange(n) * TWO_PI / n
+    tg = gs.scenario_field("taylor-green", n)
+    assert tg[5, 7] == pytest.approx(6 * np.cos(2 * x[7]) * np.cos(2 * x[5]))
+    dip = gs.scenario_field("dipole", n)
+    assert abs(dip.sum()) < 1e-6 * np.abs(dip).sum()  # equal and opposite cores
+    mer = gs.scenario_field("merger", n, {"mergerSep": 1.0})
+    assert mer.max() == pytest.approx(30.0, rel=0.1) and mer.min() >= 0
+    with pytest.raises(KeyError):
+        gs.scenario_field("nope", n)
+
+
+def test_to_half_truncates():
+    assert int(gs.to_half_truncating(np.float32(1.0))) == 0x3C00
+    # 1 + 2^-11 + 2^-12 rounds up under RNE but truncates here
+    v = np.float32(1.0 + 2.0 ** -11 + 2.0 ** -12)
+    assert int(gs.to_half_truncating(v)) == 0x3C00
+    assert int(np.float16(v).view(np.uint16)) == 0x3C01
+    assert int(gs.to_half_truncating(np.float32(-2.5))) == int(np.float16(-2.5).view(np.uint16))
+
+
+def _dye(m, seed=0):
+    rng = np.random.default_rng(seed)
+    d = np.ones((m, m, 4), np.float16)
+    d[..., :3] = rng.uniform(0, 1, (m, m, 3)).astype(np.float16)
+    return d
+
+
+def test_dye_invariants():
+    m, n = 32, 16
+    d = _dye(m)
+    zero = np.zeros((n, n, 2))
+    assert np.array_equal(gs.dye_step(d, zero, 0.01), d)
+    const = np.full((m, m, 4), 0.5, np.float16)
+    const[..., 3] = 1
+    vel = np.random.default_rng(1).normal(size=(n, n, 2))
+    assert np.allclose(gs.dye_step(const, vel, 0.01).astype(float), const.astype(float))
+    # uniform velocity moving exactly 3 texels in x: dt/2pi * u * m = 3
+    dt = 0.05
+    u = 3.0 * TWO_PI / (dt * m)
+    vel = np.zeros((n, n, 2))
+    vel[..., 0] = u
+    out = gs.dye_step(d, vel, dt)
+    assert np.allclose(out.astype(float), np.roll(d, 3, axis=1).astype(float), atol=1e-3)
+    # limiter: output within the range of the input
+    vel = np.random.default_rng(2).normal(size=(n, n, 2)) * 20
+    out = gs.dye_step(d, vel, 0.01).astype(float)
+    assert out[..., :3].min() >= d[..., :3].astype(float).min() - 1e-3
+    assert out[..., :3].max() <= d[..., :3].astype(float).max() + 1e-3

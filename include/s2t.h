@@ -180,6 +180,83 @@ int s2t_assemble(s2t_solver* h, const float* z, const float* fhat, uint32_t step
 int s2t_rhs(s2t_solver* h, uint32_t step_index, double dt, float* rhs);
 
 /* ------------------------------------------------------------------------------------------ */
+/* batched ensembles: many independent instances of mixed grid sizes stepped together           */
+/* (no counterpart in the reference, which runs one solver; SURVEY next-component f1)            */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct s2t_batch s2t_batch;
+
+/* Members 0..count-1 with grid sizes sizes[i]. Members of one size share buffers and one launch
+ * per fused kernel; each member has its own parameters, seed, clock and forcing phases, and its
+ * results are bit-identical to the same member run as a plain solver. */
+int s2t_batch_create(const int* sizes, int count, s2t_batch** out);
+int s2t_batch_destroy(s2t_batch* b);
+int s2t_batch_size(s2t_batch* b, int member, int* n);
+int s2t_batch_configure(s2t_batch* b, int member, const s2t_params* p);
+int s2t_batch_seed_spectrum(s2t_batch* b, int member, double k0, double amplitude, uint32_t seed);
+int s2t_batch_seed_physical(s2t_batch* b, int member, const float* field);
+int s2t_batch_write_state(s2t_batch* b, int member, const float* spectrum);
+int s2t_batch_read_state(s2t_batch* b, int member, float* spectrum);
+int s2t_batch_diagnostics(s2t_batch* b, int member, s2t_diag* out, double* spectrum, double* zonal);
+/* Capture each RK3 step of all members as a hipGraph (default on) or launch eagerly. */
+int s2t_batch_use_graphs(s2t_batch* b, int enable);
+/* nsteps steps of every member; enqueues only. */
+int s2t_batch_step(s2t_batch* b, int nsteps);
+int s2t_batch_sync(s2t_batch* b);
+
+/* ------------------------------------------------------------------------------------------ */
+/* serving: continuous batching of simulation requests (SURVEY next-component f2)               */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct s2t_server s2t_server;
+
+/* scenarios (src/scenarios.js:57-165, Solver.seed of solver.js:347-359) */
+#define S2T_SCENARIO_TURBULENCE 0  /* fillSpectrum(k0, amplitude, seed) */
+#define S2T_SCENARIO_TAYLOR_GREEN 1
+#define S2T_SCENARIO_DIPOLE 2
+#define S2T_SCENARIO_MERGER 3      /* scenario_param = core separation */
+
+/* request status */
+#define S2T_REQ_QUEUED 0
+#define S2T_REQ_RUNNING 1
+#define S2T_REQ_DONE 2
+#define S2T_REQ_REJECTED 3
+
+typedef struct s2t_request {
+  int n;                 /* grid size; must be one of the server's classes */
+  int steps;             /* RK3 steps to run */
+  int scenario;          /* S2T_SCENARIO_* */
+  int keep_state;        /* nonzero: keep the final spectrum for s2t_server_read_state */
+  double scenario_param; /* merger separation */
+  double k0, amplitude;  /* turbulence seeding */
+  uint32_t seed;
+  s2t_params params;
+} s2t_request;
+
+typedef struct s2t_server_stats {
+  uint64_t submitted, completed, rejected;
+  double p50_latency_s, p99_latency_s, mean_latency_s;
+  double mpts_steps_per_s; /* completed work / time from first submit to last completion */
+  double budget_bytes, used_bytes;
+} s2t_server_stats;
+
+/* Classes of slots: sizes[i] with slots[i] concurrent requests each, allocated up front and
+ * checked against budget_bytes (0 = no limit). A worker thread admits requests as slots free. */
+int s2t_server_create(const int* sizes, const int* slots, int nclasses, double budget_bytes,
+                      s2t_server** out);
+/* Stops the worker after the requests in flight; frees everything. */
+int s2t_server_destroy(s2t_server* s);
+/* Thread-safe. A size without a class is rejected (status S2T_REQ_REJECTED). */
+int s2t_server_submit(s2t_server* s, const s2t_request* r, uint64_t* id);
+/* Status, FP64 diagnostics of the final state and submit-to-completion latency (-1 until done). */
+int s2t_server_poll(s2t_server* s, uint64_t id, int* status, s2t_diag* diag, double* latency_s);
+/* Final spectrum (2*n*n floats, fixture layout) of a finished request submitted with keep_state. */
+int s2t_server_read_state(s2t_server* s, uint64_t id, float* spectrum);
+/* Wait until every submitted request is done or rejected (timeout_s <= 0: no limit). */
+int s2t_server_drain(s2t_server* s, double timeout_s);
+int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);
+
+/* ------------------------------------------------------------------------------------------ */
 /* profiling: per-kernel HIP-event timing on the solver's stream                                */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -37,7 +37,15 @@ EXPORTS = [
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
     "s2t_profile_read", "s2t_kernel_bytes",
+    "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
+    "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
+    "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
+    "s2t_batch_sync",
+    "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
+    "s2t_server_read_state", "s2t_server_drain", "s2t_server_get_stats",
 ]
+SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED = 0, 1, 2, 3
 
 
 class Params(ctypes.Structure):
@@ -57,6 +65,21 @@ class Diag(ctypes.Structure):
                 ("dt", ctypes.c_double), ("max_speed", ctypes.c_double),
                 ("time", ctypes.c_double), ("step_index", ctypes.c_uint32),
                 ("nonfinite", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("steps", ctypes.c_int), ("scenario", ctypes.c_int),
+                ("keep_state", ctypes.c_int), ("scenario_param", ctypes.c_double),
+                ("k0", ctypes.c_double), ("amplitude", ctypes.c_double), ("seed", ctypes.c_uint32),
+                ("params", Params)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double),
+                ("p99_latency_s", ctypes.c_double), ("mean_latency_s", ctypes.c_double),
+                ("mpts_steps_per_s", ctypes.c_double), ("budget_bytes", ctypes.c_double),
+                ("used_bytes", ctypes.c_double)]
 
 
 class S2TError(RuntimeError):
@@ -108,6 +131,27 @@ def load(path=None):
         "s2t_profile": (ip, [vp, ip]),
         "s2t_profile_read": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp)]),
         "s2t_kernel_bytes": (ip, [vp, ip, ip, ctypes.POINTER(dp)]),
+        "s2t_batch_create": (ip, [ctypes.POINTER(ctypes.c_int), ip, ctypes.POINTER(vp)]),
+        "s2t_batch_destroy": (ip, [vp]),
+        "s2t_batch_size": (ip, [vp, ip, ctypes.POINTER(ip)]),
+        "s2t_batch_configure": (ip, [vp, ip, ctypes.POINTER(Params)]),
+        "s2t_batch_seed_spectrum": (ip, [vp, ip, dp, dp, u32]),
+        "s2t_batch_seed_physical": (ip, [vp, ip, fp]),
+        "s2t_batch_write_state": (ip, [vp, ip, fp]),
+        "s2t_batch_read_state": (ip, [vp, ip, fp]),
+        "s2t_batch_diagnostics": (ip, [vp, ip, ctypes.POINTER(Diag), fp, fp]),
+        "s2t_batch_use_graphs": (ip, [vp, ip]),
+        "s2t_batch_step": (ip, [vp, ip]),
+        "s2t_batch_sync": (ip, [vp]),
+        "s2t_server_create": (ip, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ip,
+                                   dp, ctypes.POINTER(vp)]),
+        "s2t_server_destroy": (ip, [vp]),
+        "s2t_server_submit": (ip, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+        "s2t_server_poll": (ip, [vp, ctypes.c_uint64, ctypes.POINTER(ip), ctypes.POINTER(Diag),
+                                 ctypes.POINTER(dp)]),
+        "s2t_server_read_state": (ip, [vp, ctypes.c_uint64, fp]),
+        "s2t_server_drain": (ip, [vp, dp]),
+        "s2t_server_get_stats": (ip, [vp, ctypes.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -301,3 +345,157 @@ class Solver:
         b = ctypes.c_double()
         _check(self.lib, self.lib.s2t_kernel_bytes(self.h, kind, stage, ctypes.byref(b)))
         return b.value
+
+
+def _params(kw):
+    p = dict(DEFAULTS)
+    unknown = set(kw) - set(DEFAULTS)
+    if unknown:
+        raise KeyError("unknown parameters: %s" % sorted(unknown))
+    p.update(kw)
+    return Params(**{k: float(v) for k, v in p.items()})
+
+
+def _diag_dict(d, spec=None, zonal=None):
+    out = dict(energy=d.energy, enstrophy=d.enstrophy, dt=d.dt, maxSpeed=d.max_speed,
+               time=d.time, stepIndex=d.step_index, nonfinite=d.nonfinite)
+    if spec is not None:
+        out["spectrum"] = spec
+        out["zonal"] = zonal
+    return out
+
+
+class Batch:
+    """Ensemble of independent members of mixed grid sizes (s2t_batch_*)."""
+
+    def __init__(self, sizes):
+        self.lib = load()
+        self.sizes = [int(n) for n in sizes]
+        arr = (ctypes.c_int * len(self.sizes))(*self.sizes)
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.s2t_batch_create(arr, len(self.sizes), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.s2t_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, i, **kw):
+        p = _params(kw)
+        _check(self.lib, self.lib.s2t_batch_configure(self.h, i, ctypes.byref(p)))
+
+    def seed_spectrum(self, i, k0=14.0, amplitude=6.0, seed=1):
+        _check(self.lib, self.lib.s2t_batch_seed_spectrum(self.h, i, float(k0), float(amplitude),
+                                                          int(seed) & 0xFFFFFFFF))
+
+    def seed_physical(self, i, field):
+        f = np.ascontiguousarray(np.asarray(field, np.float32).reshape(-1))
+        _check(self.lib, self.lib.s2t_batch_seed_physical(self.h, i, _ptr(f)))
+
+    def write_state(self, i, w):
+        n = self.sizes[i]
+        a = to_interleaved(np.asarray(w).reshape(n, n))
+        _check(self.lib, self.lib.s2t_batch_write_state(self.h, i, _ptr(a)))
+
+    def read_state(self, i):
+        n = self.sizes[i]
+        a = np.empty(2 * n * n, np.float32)
+        _check(self.lib, self.lib.s2t_batch_read_state(self.h, i, _ptr(a)))
+        return from_interleaved(a, n)
+
+    def diagnostics(self, i):
+        n = self.sizes[i]
+        d = Diag()
+        spec = np.empty(n // 2 + 1, np.float64)
+        zonal = np.empty(n, np.float64)
+        _check(self.lib, self.lib.s2t_batch_diagnostics(self.h, i, ctypes.byref(d), _ptr(spec),
+                                                        _ptr(zonal)))
+        return _diag_dict(d, spec, zonal)
+
+    def use_graphs(self, enable):
+        _check(self.lib, self.lib.s2t_batch_use_graphs(self.h, 1 if enable else 0))
+
+    def step(self, nsteps=1):
+        _check(self.lib, self.lib.s2t_batch_step(self.h, int(nsteps)))
+
+    def sync(self):
+        _check(self.lib, self.lib.s2t_batch_sync(self.h))
+
+
+class Server:
+    """Continuous-batching server (s2t_server_*): classes of slots per grid size."""
+
+    def __init__(self, classes, budget_bytes=0.0):
+        self.lib = load()
+        sizes = sorted(classes)
+        self.sizes = sizes
+        a = (ctypes.c_int * len(sizes))(*sizes)
+        b = (ctypes.c_int * len(sizes))(*[int(classes[n]) for n in sizes])
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.s2t_server_create(a, b, len(sizes), float(budget_bytes),
+                                                    ctypes.byref(h)))
+        self.h = h
+        self.n_of = {}
+
+    def close(self):
+        if self.h:
+            self.lib.s2t_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, n, steps, scenario="turbulence", scenario_param=0.0, k0=14.0, amplitude=6.0,
+               seed=1, keep_state=False, **params):
+        r = Request(n=int(n), steps=int(steps), scenario=SCENARIO[scenario],
+                    keep_state=1 if keep_state else 0, scenario_param=float(scenario_param),
+                    k0=float(k0), amplitude=float(amplitude), seed=int(seed) & 0xFFFFFFFF,
+                    params=_params(params))
+        rid = ctypes.c_uint64()
+        _check(self.lib, self.lib.s2t_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        self.n_of[rid.value] = int(n)
+        return rid.value
+
+    def poll(self, rid):
+        st = ctypes.c_int()
+        d = Diag()
+        lat = ctypes.c_double()
+        _check(self.lib, self.lib.s2t_server_poll(self.h, ctypes.c_uint64(rid), ctypes.byref(st),
+                                                  ctypes.byref(d), ctypes.byref(lat)))
+        return st.value, _diag_dict(d), lat.value
+
+    def read_state(self, rid):
+        n = self.n_of[rid]
+        a = np.empty(2 * n * n, np.float32)
+        _check(self.lib, self.lib.s2t_server_read_state(self.h, ctypes.c_uint64(rid), _ptr(a)))
+        return from_interleaved(a, n)
+
+    def drain(self, timeout_s=0.0):
+        _check(self.lib, self.lib.s2t_server_drain(self.h, float(timeout_s)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(self.lib, self.lib.s2t_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

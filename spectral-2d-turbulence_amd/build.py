@@ -19,7 +19,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(PKG, "_build")
-SOURCES = ["step_kernels.hip", "aux_kernels.hip", "api.hip"]
+SOURCES = ["step_kernels.hip", "aux_kernels.hip", "core.hip", "api.hip", "batch.hip",
+           "server.hip"]
 ARCH = os.environ.get("S2T_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -43,17 +43,44 @@ void fill_u32(hipStream_t st, void* p, size_t words, uint32_t v) {
   check_launch("k_fill_u32");
 }
 
-__global__ void k_reset_clock(DevState* d, double dt_max) {
-  d->dt = dt_max;
-  d->t = 0.0;
-  d->speed = 0.f;
-  d->speed_bits = 0u;
-  d->step = 0xFFFFFFFFu;
-  d->nonfinite = 0u;
+// zero the clock of instances i0 .. i0+ni-1 and make them active (part of Solver.reset,
+// solver.js:420-444: stepIndex = 0, simTime = 0, dt = dtMax)
+__global__ void k_reset_inst(Inst* in, int i0, int ni) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  DevState& d = in[i0 + i].st;
+  d.dt = in[i0 + i].ph.dt_max;
+  d.t = 0.0;
+  d.speed = 0.f;
+  d.speed_bits = 0u;
+  d.step = 0u;
+  d.nonfinite = 0u;
+  d.active = 1;
+  d.steps_left = -1;
 }
-void reset_clock(hipStream_t st, DevState* d, double dt_max) {
-  hipLaunchKernelGGL(k_reset_clock, dim3(1), dim3(1), 0, st, d, dt_max);
-  check_launch("k_reset_clock");
+void reset_inst(hipStream_t st, Inst* in, int i0, int ni) {
+  hipLaunchKernelGGL(k_reset_inst, dim3(blocks(ni, 64)), dim3(64), 0, st, in, i0, ni);
+  check_launch("k_reset_inst");
+}
+
+// per-step epilogue: advance stepIndex and time, clear the speed maximum for the next step and
+// retire instances whose step budget is spent (stepIndex++ of solver.js:520)
+__global__ void k_step_end(Inst* in, int ni) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  DevState& d = in[i].st;
+  if (!d.active) return;
+  d.step += 1u;
+  d.t += d.dt;
+  d.speed_bits = 0u;
+  if (d.steps_left > 0) {
+    d.steps_left -= 1;
+    if (d.steps_left == 0) d.active = 0;
+  }
+}
+void step_end(hipStream_t st, Inst* in, int ni) {
+  hipLaunchKernelGGL(k_step_end, dim3(blocks(ni, 64)), dim3(64), 0, st, in, ni);
+  check_launch("k_step_end");
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -1,0 +1,358 @@
+// Slot-aware solver internals behind the C ABI: allocation, configuration, seeding, state I/O,
+// diagnostics and step scheduling. Mirrors the buffer ownership and operations of the WebGPU
+// Solver (src/gpu/solver.js:77-608).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "aux.hpp"
+#include "core.hpp"
+
+namespace s2t {
+
+void hipchk(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory) throw Fail(S2T_ENOMEM, std::string(what) + ": out of device memory");
+  throw Fail(S2T_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void check_launch(const char* what) { hipchk(hipGetLastError(), what); }
+
+template <class T>
+static T* dalloc(size_t elems, const char* what) {
+  void* p = nullptr;
+  hipchk(hipMalloc(&p, elems * sizeof(T) + 256), what);
+  return (T*)p;
+}
+
+// ---------------------------------------------------------------------------------------------
+// profiling: HIP events recorded on the solver's stream around each launch
+// ---------------------------------------------------------------------------------------------
+void prof_begin(Solver* s, int kind) {
+  if (!s->prof) return;
+  if (s->nev == s->cap_ev) {
+    const int cap = s->cap_ev ? s->cap_ev * 2 : 256;
+    Solver::Ev* ev = (Solver::Ev*)realloc(s->evs, sizeof(Solver::Ev) * cap);
+    if (!ev) throw Fail(S2T_ENOMEM, "profiling event list");
+    for (int i = s->cap_ev; i < cap; ++i) {
+      hipchk(hipEventCreate(&ev[i].a), "hipEventCreate");
+      hipchk(hipEventCreate(&ev[i].b), "hipEventCreate");
+    }
+    s->evs = ev;
+    s->cap_ev = cap;
+  }
+  s->evs[s->nev].kind = kind;
+  hipchk(hipEventRecord(s->evs[s->nev].a, s->stream), "hipEventRecord");
+}
+
+void prof_end(Solver* s) {
+  if (!s->prof) return;
+  hipchk(hipEventRecord(s->evs[s->nev].b, s->stream), "hipEventRecord");
+  s->nev += 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------------------------
+int radix(int n, int* stages, int* r2) {
+  if (n < 1) return -1;
+  int st = 0, m = n;
+  while (m % 4 == 0) {
+    ++st;
+    m /= 4;
+  }
+  if (m != 1 && m != 2) return -1;
+  if (stages) *stages = st;
+  if (r2) *r2 = m == 2;
+  return 0;
+}
+
+std::string size_error(int n) {
+  if (radix(n, nullptr, nullptr) != 0)
+    return "grid size " + std::to_string(n) + " must be 4^a or 2 * 4^a";  // fft.js:28
+  if (n < 16 || n > 4096) return "grid size " + std::to_string(n) + " outside 16..4096";
+  return "";
+}
+
+// defaults of the reference's config object (src/main.js:9-36)
+s2t_params default_params() {
+  s2t_params p{};
+  p.beta = 0;
+  p.drag = 0.01;
+  p.viscosity = 0.67;
+  p.hyper_p = 4;
+  p.force_amp = 0;
+  p.force_k = 22;
+  p.force_width = 2.5;
+  p.cfl = 0.5;
+  p.dt_min = 1e-5;
+  p.dt_max = 0.02;
+  p.dye_fade = 0.999;
+  p.fixed_dt = 0;
+  return p;
+}
+
+std::string check_params(const s2t_params& p) {
+  if (!(p.hyper_p > 0)) return "hyper_p must be positive";
+  if (p.force_amp > 0 && !(p.force_width > 0)) return "force_width must be positive";
+  if (!(p.dt_max > 0)) return "dt_max must be positive";
+  return "";
+}
+
+// Solver.configure (solver.js:306-324): nu_h = viscosity / kcut^(2p), FORCE_NORM = 0.01 N^2
+static Phys make_phys(int n, const s2t_params& p) {
+  Phys ph{};
+  const double kcut = n / 3.0;
+  ph.beta = p.beta;
+  ph.mu = p.drag;
+  ph.p = p.hyper_p;
+  ph.nu = p.viscosity > 0 ? p.viscosity / std::pow(kcut, 2.0 * p.hyper_p) : 0.0;
+  const double ip = std::floor(p.hyper_p);
+  ph.p_int = (ip == p.hyper_p && ip >= 1 && ip <= 16) ? (int)ip : 0;
+  ph.force_amp = (float)p.force_amp;
+  ph.force_k = (float)p.force_k;
+  ph.force_width = (float)p.force_width;
+  ph.force_norm = (float)(0.01 * n * n);  // src/gpu/shaders.js:26
+  ph.cfl = p.cfl;
+  ph.dt_min = p.dt_min;
+  ph.dt_max = p.dt_max;
+  ph.dx = 2.0 * M_PI / n;
+  ph.fixed_dt = p.fixed_dt;
+  ph.dye_fade = (float)p.dye_fade;
+  return ph;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host <-> device helpers on the solver's stream
+// ---------------------------------------------------------------------------------------------
+static void h2d(Solver* s, void* dst, const void* src, size_t bytes) {
+  hipchk(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), "hipMemcpyAsync H2D");
+  hipchk(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+static void d2h(Solver* s, void* dst, const void* src, size_t bytes) {
+  hipchk(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream), "hipMemcpyAsync D2H");
+  hipchk(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+void sync(Solver* s) { hipchk(hipStreamSynchronize(s->stream), "hipStreamSynchronize"); }
+
+static void need_full(Solver* s) {
+  // three full fields plus one compact state (the fused rhs hook)
+  if (!s->full)
+    s->full = dalloc<float2>((size_t)s->n * s->n * 3 + s->ops->state_elems, "full scratch");
+  s->full2 = s->full + (size_t)s->n * s->n;
+}
+static void need_force(Solver* s) {
+  if (!s->mixed_f) s->mixed_f = dalloc<float2>(s->ops->plane_elems * s->batch, "force tiles");
+  if (!s->force_k) s->force_k = dalloc<float2>(s->ops->state_elems * s->batch, "force spectrum");
+}
+static void need_vel(Solver* s) {
+  if (!s->vel) {
+    s->vel = dalloc<float2>((size_t)s->n * s->n, "velocity");
+    fill_u32(s->stream, s->vel, (size_t)s->n * s->n * 2, 0u);
+  }
+}
+
+static float2* slot_state(Solver* s, int slot) {
+  return s->state[s->phase] + (size_t)slot * s->ops->state_elems;
+}
+
+// ---------------------------------------------------------------------------------------------
+// lifecycle (Solver.create / build / destroy, solver.js:71-140, 614-629)
+// ---------------------------------------------------------------------------------------------
+Solver* solver_new(int n, int dye_scale, int batch, bool with_dye) {
+  Solver* s = new Solver();
+  try {
+    s->n = n;
+    s->K = n / 3;
+    s->dye_scale = dye_scale;
+    s->m = n * dye_scale;
+    s->batch = batch;
+    s->with_dye = with_dye;
+    s->ops = ops_for(n);
+    hipchk(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t st = s->ops->state_elems * batch, pl = s->ops->plane_elems * batch;
+    for (int i = 0; i < 3; ++i) s->state[i] = dalloc<float2>(st, "state");
+    s->mixed_u = dalloc<float2>(3 * pl, "mixed omega/u/v");
+    s->mixed_z = dalloc<float2>(2 * pl, "mixed products");
+    s->inst = dalloc<Inst>(batch, "instance scalars");
+    const int nb = n / 2 + 1;
+    s->diag_bins = dalloc<double>(2 * nb + 2, "diagnostic bins");
+    s->diag_zonal = dalloc<double>(n, "zonal mean");
+    s->seed_sum = dalloc<double>(s->K + 2, "seed sums");
+    if (with_dye)
+      for (int i = 0; i < 3; ++i) s->dye[i] = dalloc<ushort4>((size_t)s->m * s->m, "dye");
+    // twiddles in FP64, rounded once
+    std::vector<float2> tw(s->ops->twiddle_elems > 0 ? s->ops->twiddle_elems : 1);
+    s->ops->twiddles(tw.data());
+    s->tw = dalloc<float2>(tw.size(), "twiddles");
+    h2d(s, s->tw, tw.data(), tw.size() * sizeof(float2));
+    std::vector<double2> wt(n);
+    for (int k = 0; k < n; ++k) {
+      const double a = 2.0 * M_PI * k / n;
+      wt[k] = make_double2(std::cos(a), std::sin(a));
+    }
+    s->wtab = dalloc<double2>(n, "zonal table");
+    h2d(s, s->wtab, wt.data(), n * sizeof(double2));
+    s->phys_slot.assign(batch, make_phys(n, default_params()));
+    s->params_slot.assign(batch, default_params());
+    s->primed_slot.assign(batch, 0);
+    for (int i = 0; i < 3; ++i) fill_u32(s->stream, s->state[i], st * 2, 0u);
+    if (with_dye)
+      for (int i = 0; i < 3; ++i) fill_u32(s->stream, s->dye[i], (size_t)s->m * s->m * 2, 0u);
+    fill_u32(s->stream, s->mixed_u, 3 * pl * 2, 0u);
+    for (int i = 0; i < batch; ++i) {
+      hipchk(hipMemcpyAsync(&s->inst[i].ph, &s->phys_slot[i], sizeof(Phys), hipMemcpyHostToDevice,
+                            s->stream),
+             "hipMemcpyAsync params");
+    }
+    reset_inst(s->stream, s->inst, 0, batch);
+    sync(s);
+    return s;
+  } catch (...) {
+    solver_free(s);
+    throw;
+  }
+}
+
+void solver_free(Solver* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void* bufs[] = {s->state[0], s->state[1], s->state[2], s->mixed_u,  s->mixed_z,    s->mixed_f,
+                  s->force_k,  s->vel,      s->full,     s->tw,       s->wtab,       s->inst,
+                  s->diag_bins, s->diag_zonal, s->seed_sum, s->dye[0], s->dye[1], s->dye[2]};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int i = 0; i < s->cap_ev; ++i) {
+    (void)hipEventDestroy(s->evs[i].a);
+    (void)hipEventDestroy(s->evs[i].b);
+  }
+  free(s->evs);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-slot operations
+// ---------------------------------------------------------------------------------------------
+void configure_slot(Solver* s, int slot, const s2t_params& p) {
+  s->params_slot[slot] = p;
+  s->phys_slot[slot] = make_phys(s->n, p);
+  h2d(s, &s->inst[slot].ph, &s->phys_slot[slot], sizeof(Phys));
+}
+
+void reset_slot(Solver* s, int slot) {
+  reset_inst(s->stream, s->inst, slot, 1);
+  s->primed_slot[slot] = 0;
+}
+
+void set_budget_slot(Solver* s, int slot, int steps) {
+  h2d(s, &s->inst[slot].st.steps_left, &steps, sizeof(int));
+}
+
+DevState read_clock_slot(Solver* s, int slot) {
+  DevState d{};
+  d2h(s, &d, &s->inst[slot].st, sizeof(DevState));
+  return d;
+}
+
+// Solver.seed via fillSpectrum (solver.js:347-359, 384-418), on the device
+void seed_spectrum_slot(Solver* s, int slot, double k0, double amplitude, uint32_t seed) {
+  seed_spectrum(s->stream, slot_state(s, slot), s->seed_sum, s->n, k0, amplitude, seed);
+  reset_slot(s, slot);
+  sync(s);
+}
+
+// Solver.seedPhysical (solver.js:361-382): forward FFT, 2/3 mask, k = 0 zeroed
+void seed_physical_slot(Solver* s, int slot, const float* field) {
+  need_full(s);
+  const size_t nn = (size_t)s->n * s->n;
+  float* tmp = reinterpret_cast<float*>(s->full2);
+  h2d(s, tmp, field, nn * sizeof(float));
+  real_to_complex(s->stream, tmp, s->full, nn);
+  s->ops->fft2(s, s->full, 0);
+  s->ops->pack(s, s->full, slot_state(s, slot));
+  reset_slot(s, slot);
+  sync(s);
+}
+
+void write_state_slot(Solver* s, int slot, const float* spectrum) {
+  need_full(s);
+  h2d(s, s->full, spectrum, (size_t)s->n * s->n * sizeof(float2));
+  s->ops->pack(s, s->full, slot_state(s, slot));
+  s->primed_slot[slot] = 0;
+  sync(s);
+}
+
+void read_state_slot(Solver* s, int slot, float* spectrum) {
+  need_full(s);
+  s->ops->unpack(s, slot_state(s, slot), s->full);
+  d2h(s, spectrum, s->full, (size_t)s->n * s->n * sizeof(float2));
+}
+
+// readDiagnostics (solver.js:547-608) in FP64 with a fixed reduction order
+void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal) {
+  const int nb = s->n / 2 + 1;
+  diagnostics(s->stream, slot_state(s, slot), s->wtab, s->diag_bins, s->diag_zonal, s->n);
+  std::vector<double> bins(2 * nb + 2);
+  d2h(s, bins.data(), s->diag_bins, bins.size() * sizeof(double));
+  const DevState d = read_clock_slot(s, slot);
+  if (out) {
+    out->energy = bins[2 * nb];
+    out->enstrophy = bins[2 * nb + 1];
+    out->dt = d.dt;
+    out->max_speed = d.speed;
+    out->time = d.t;
+    out->step_index = d.step;
+    out->nonfinite = d.nonfinite;
+  }
+  if (spectrum) memcpy(spectrum, bins.data(), nb * sizeof(double));
+  if (zonal) d2h(s, zonal, s->diag_zonal, s->n * sizeof(double));
+}
+
+// ---------------------------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------------------------
+// the velocity prime (encodePrime, solver.js:475-486) of every slot whose state was replaced
+void ensure_primed(Solver* s) {
+  int i = 0;
+  while (i < s->batch) {
+    if (s->primed_slot[i]) {
+      ++i;
+      continue;
+    }
+    int j = i;
+    while (j < s->batch && !s->primed_slot[j]) ++j;
+    s->ops->prime(s, i, j - i);
+    for (int k = i; k < j; ++k) s->primed_slot[k] = 1;
+    i = j;
+  }
+}
+
+// encodeStep (+ encodeReduce, + encodeDye) of solver.js:488-545 for all active slots
+void step_once(Solver* s, bool dye) {
+  const bool force = s->splats.count > 0;
+  if (force) need_force(s);
+  if (dye) need_vel(s);
+  ensure_primed(s);
+  s->ops->rows(s, 0, true, dye, force);
+  s->ops->cols(s, 0, force);
+  s->ops->rows(s, 1, false, false, false);
+  s->ops->cols(s, 1, force);
+  s->ops->rows(s, 2, false, false, false);
+  s->ops->cols(s, 2, force);
+  s->phase = (s->phase + 1) % 3;
+  if (dye) {
+    prof_begin(s, S2T_PROF_DYE);
+    dye_step(s->stream, s->dye, s->dye_parity, s->vel, &s->inst[0].st, 0.0,
+             s->phys_slot[0].dye_fade, s->splats, s->n, s->m);
+    prof_end(s);
+    s->dye_parity = 1 - s->dye_parity;
+  }
+  prof_begin(s, S2T_PROF_OTHER);
+  step_end(s->stream, s->inst, s->batch);
+  prof_end(s);
+}
+
+}  // namespace s2t

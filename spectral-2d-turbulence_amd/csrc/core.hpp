@@ -1,0 +1,45 @@
+// Internal solver operations shared by the plain C ABI (api.hip) and the batch / serving C ABI
+// (batch.hip). A Solver holds `batch` instances ("slots") of one grid size; every operation on
+// one instance names its slot, and stepping advances all active slots with one launch per kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/s2t.h"
+#include "solver.hpp"
+
+namespace s2t {
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipchk(hipError_t e, const char* what);
+// 0 when n is 4^a or 2*4^a (src/gpu/fft.js:20-31)
+int radix(int n, int* stages, int* has_radix2);
+std::string size_error(int n);  // "" when n is a supported grid size
+s2t_params default_params();
+std::string check_params(const s2t_params& p);  // "" when valid
+
+Solver* solver_new(int n, int dye_scale, int batch, bool with_dye);
+void solver_free(Solver* s);
+
+void configure_slot(Solver* s, int slot, const s2t_params& p);
+void reset_slot(Solver* s, int slot);
+void seed_spectrum_slot(Solver* s, int slot, double k0, double amplitude, uint32_t seed);
+void seed_physical_slot(Solver* s, int slot, const float* field);
+void write_state_slot(Solver* s, int slot, const float* spectrum);
+void read_state_slot(Solver* s, int slot, float* spectrum);
+void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal);
+// limit the slot to `steps` more steps, after which it retires itself (steps < 0: no limit)
+void set_budget_slot(Solver* s, int slot, int steps);
+DevState read_clock_slot(Solver* s, int slot);
+
+void ensure_primed(Solver* s);
+void step_once(Solver* s, bool dye);
+void sync(Solver* s);
+
+}  // namespace s2t

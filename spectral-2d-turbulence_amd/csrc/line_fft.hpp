@@ -211,16 +211,20 @@ template <int N, int DIR>
 __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds, int t,
                                          const float2* __restrict__ tw) {
   using L = Line<N>;
+  // Opaque copy of the table pointer: keeps the compiler from hoisting every pass's twiddle loads
+  // out of the callers' field loops, which would pin ~60 VGPRs for the whole kernel.
+  const float2* twl = tw;
+  asm volatile("" : "+s"(twl));
   dft<L::R0, DIR>(v);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < L::R0; ++r) lds[pad(t * L::R0 + r)] = v[r];
   __syncthreads();
   if constexpr (L::NPASS == 2) {
-    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, tw + L::TW1);
+    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, twl + L::TW1);
   } else {
-    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, tw + L::TW1);
-    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, tw + L::TW2);
+    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, twl + L::TW1);
+    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, twl + L::TW2);
   }
 }
 

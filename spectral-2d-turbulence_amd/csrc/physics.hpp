@@ -32,9 +32,11 @@ struct Geo {
   static __device__ __forceinline__ int jrow(int i) {
     return i <= K ? i : (i >= N - K ? i - (N - NJ) : -1);
   }
+  // 32-bit element offsets (every buffer here holds fewer than 2^31 elements), so loads and
+  // stores take a uniform base plus one offset VGPR
   template <int F>
-  static __device__ __forceinline__ size_t tile(int y, int kx, int f) {
-    return ((size_t)((y >> 2) * KC + kx) * F + f) * 4 + (y & 3);
+  static __device__ __forceinline__ unsigned tile(int y, int kx, int f) {
+    return ((unsigned)((y >> 2) * KC + kx) * F + f) * 4u + (unsigned)(y & 3);
   }
 };
 
@@ -60,15 +62,26 @@ struct Splats {
   Splat s[8];
 };
 
-// Per-solver device scalars. dt and dt-derived values never leave the device during stepping
-// (README.md:132-134 of the reference); the host reads them with the diagnostics.
+// Per-instance device scalars. dt and dt-derived values never leave the device during stepping
+// (README.md:132-134 of the reference); the host reads them with the diagnostics. Only the
+// per-step epilogue kernel (k_step_end) advances step, t and steps_left, so no kernel reads a
+// field that another block of the same launch writes.
 struct DevState {
   double dt;            // step of the step in flight / last step
   double t;             // simulated time
   float speed;          // max |u|+|v| of the last step's initial field
   unsigned speed_bits;  // running max for the step in flight (float bits, non-negative)
-  unsigned step;        // stepIndex of the step in flight (all ones before the first step)
+  unsigned step;        // steps completed since the last reset = stepIndex of the step in flight
   unsigned nonfinite;   // sticky flag: a non-finite speed was seen
+  int active;           // 0: the step kernels skip this instance
+  int steps_left;       // steps until the instance retires itself (< 0: no limit)
+};
+
+// One simulation instance's parameters and scalars, an array of these per solver (a plain solver
+// has one; a batch has one per member). Kernels index it with blockIdx.y.
+struct Inst {
+  Phys ph;
+  DevState st;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -136,6 +149,69 @@ __device__ __forceinline__ double cfl_dt(float speed, const Phys& ph) {
   return fmax(dt, ph.dt_min);
 }
 
+// FP64 exp and sincos for the integrating factor, a few double ulp accurate, so the factor is
+// still correctly rounded once to FP32. Method: Cody-Waite reduction by ln2 (resp. pi/2) with
+// split constants (high parts carry 32-33 fractional bits so n*hi is exact), then the Taylor
+// series of exp to r^12 on |r| <= ln2/2 and of sin / cos to r^15 / r^16 on |r| <= pi/4 (Horner).
+// Arguments outside the reduction's exact range fall back to the library routines.
+__host__ __device__ __forceinline__ double exp_f64(double y) {
+  if (!(y > -745.0)) return y == y ? 0.0 : y;
+  if (!(y < 709.0)) return exp(y);
+  const double n = rint(y * 1.4426950408889634);
+  double r = fma(-n, 0.6931471806019545, y);
+  r = fma(-n, -4.2009150726810846e-11, r);
+  double p = 2.08767569878681e-09;
+  p = fma(p, r, 2.505210838544172e-08);
+  p = fma(p, r, 2.755731922398589e-07);
+  p = fma(p, r, 2.7557319223985893e-06);
+  p = fma(p, r, 2.48015873015873e-05);
+  p = fma(p, r, 0.0001984126984126984);
+  p = fma(p, r, 0.001388888888888889);
+  p = fma(p, r, 0.008333333333333333);
+  p = fma(p, r, 0.041666666666666664);
+  p = fma(p, r, 0.16666666666666666);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)n);
+}
+
+__host__ __device__ __forceinline__ void sincos_f64(double x, double* s, double* c) {
+  if (!(fabs(x) < 1.0e5)) {
+    *s = sin(x);
+    *c = cos(x);
+    return;
+  }
+  const double q = rint(x * 0.6366197723675814);
+  double r = fma(-q, 1.570796326850541, x);
+  r = fma(-q, -5.5644316763895155e-11, r);
+  r = fma(-q, 2.0222662487959506e-21, r);
+  const double r2 = r * r;
+  double ps = -7.647163731819816e-13;
+  ps = fma(ps, r2, 1.6059043836821613e-10);
+  ps = fma(ps, r2, -2.505210838544172e-08);
+  ps = fma(ps, r2, 2.7557319223985893e-06);
+  ps = fma(ps, r2, -0.0001984126984126984);
+  ps = fma(ps, r2, 0.008333333333333333);
+  ps = fma(ps, r2, -0.16666666666666666);
+  const double sr = fma(ps * r2, r, r);
+  double pc = 4.779477332387385e-14;
+  pc = fma(pc, r2, -1.1470745597729725e-11);
+  pc = fma(pc, r2, 2.08767569878681e-09);
+  pc = fma(pc, r2, -2.755731922398589e-07);
+  pc = fma(pc, r2, 2.48015873015873e-05);
+  pc = fma(pc, r2, -0.001388888888888889);
+  pc = fma(pc, r2, 0.041666666666666664);
+  pc = fma(pc, r2, -0.5);
+  const double cr = fma(pc, r2, 1.0);
+  switch (((long long)q) & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
+}
+
 // Integrating factor E(s) = exp(L s), L = -mu - nu (k^2)^p + i beta kx / k^2
 // (reference/spectral2d.py:26, solver.wgsl expL:97-101), evaluated in FP64 once per mode and
 // stage as E(dt/2); E(dt) = E(dt/2)^2 and E(-dt/2) = 1/E(dt/2).
@@ -151,38 +227,37 @@ __device__ __forceinline__ IFactor ifactor_half(double kx, double k2, const Phys
     k2p = pow(k2, ph.p);
   }
   const double re = -ph.mu - ph.nu * k2p;
-  const double im = ph.beta * kx / k2;
   IFactor e;
-  e.mag = exp(re * 0.5 * dt);
-  sincos(im * 0.5 * dt, &e.s, &e.c);
+  e.mag = exp_f64(re * 0.5 * dt);
+  if (ph.beta != 0.0) {
+    sincos_f64(ph.beta * kx / k2 * 0.5 * dt, &e.s, &e.c);
+  } else {
+    e.s = 0.0;
+    e.c = 1.0;
+  }
   return e;
 }
 
 // RK coefficients [cA, a, cB, b] of the three IF-SSP-RK3 stages (src/gpu/solver.js:11-15):
 //   wout = cA E(a dt) w0 + cB E(b dt) (wcur + dt rhs)
-// evaluated in FP64 and rounded once.
+// evaluated in FP64 from the FP64 E(dt/2) and rounded once: rounding E to FP32 instead would bias
+// a pure decay by ~2e-8 per step, the same every step (2e-5 over 1000 steps).
 __device__ __forceinline__ float2 rk_update(int stage, const IFactor& e, float2 w0, float2 wcur,
                                             float2 rhs, double dt) {
   const double ax = (double)wcur.x + dt * (double)rhs.x;
   const double ay = (double)wcur.y + dt * (double)rhs.y;
+  const double hr = e.mag * e.c, hi = e.mag * e.s;  // E(dt/2)
   if (stage == 0) {
-    // E(dt) (w0 + dt f0)
-    const double m2 = e.mag * e.mag;
-    const double er = m2 * (e.c * e.c - e.s * e.s), ei = m2 * (2.0 * e.c * e.s);
+    const double er = hr * hr - hi * hi, ei = 2.0 * hr * hi;  // E(dt)
     return make_float2((float)(er * ax - ei * ay), (float)(er * ay + ei * ax));
   }
   if (stage == 1) {
-    // 3/4 E(dt/2) w0 + 1/4 E(-dt/2) (w1 + dt f1)
-    const double hr = e.mag * e.c, hi = e.mag * e.s;
-    const double ir = e.c / e.mag, ii = -e.s / e.mag;
+    const double ir = e.c / e.mag, ii = -e.s / e.mag;  // E(-dt/2)
     const double x = 0.75 * (hr * w0.x - hi * w0.y) + 0.25 * (ir * ax - ii * ay);
     const double y = 0.75 * (hr * w0.y + hi * w0.x) + 0.25 * (ir * ay + ii * ax);
     return make_float2((float)x, (float)y);
   }
-  // 1/3 E(dt) w0 + 2/3 E(dt/2) (w2 + dt f2)
-  const double m2 = e.mag * e.mag;
-  const double er = m2 * (e.c * e.c - e.s * e.s), ei = m2 * (2.0 * e.c * e.s);
-  const double hr = e.mag * e.c, hi = e.mag * e.s;
+  const double er = hr * hr - hi * hi, ei = 2.0 * hr * hi;
   const double x = (1.0 / 3.0) * (er * w0.x - ei * w0.y) + (2.0 / 3.0) * (hr * ax - hi * ay);
   const double y = (1.0 / 3.0) * (er * w0.y + ei * w0.x) + (2.0 / 3.0) * (hr * ay + hi * ax);
   return make_float2((float)x, (float)y);

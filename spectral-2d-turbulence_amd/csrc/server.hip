@@ -1,0 +1,444 @@
+// Continuous-batching serving loop (C ABI s2t_server_*).
+//
+// Requests (grid size, step count, scenario, parameters) arrive from any thread. A native worker
+// thread admits each into a free slot of the class of its grid size, seeds it on the device and
+// sets its step budget; every active class is then stepped on its own stream (one launch per
+// fused kernel covers all slots of a class; idle slots exit at once). Each slot retires itself
+// on the device after exactly its budgeted steps (k_step_end), so a result does not depend on
+// when the host polls or on which other requests share the class: it is bit-identical to the same
+// request run alone. Completions record their latency and FP64 diagnostics (and, on request, the
+// final spectrum); the server reports p50/p99 latency and aggregate throughput.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/s2t.h"
+#include "aux.hpp"
+#include "core.hpp"
+
+namespace s2t {
+
+extern thread_local std::string g_err;
+int api_bad(const std::string& m);
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------------------------
+// scenarios on the host: physical vorticity on x_j = 2 pi j / N (src/scenarios.js:57-165)
+// ---------------------------------------------------------------------------------------------
+static double periodic_delta(double a, double b) {
+  double d = a - b;
+  if (d > M_PI) d -= 2 * M_PI;
+  else if (d < -M_PI) d += 2 * M_PI;
+  return d;
+}
+
+struct Core {
+  double x, y, a, amp;
+};
+
+static void gaussian_cores(int n, const std::vector<Core>& cores, std::vector<float>& f) {
+  const double h = 2 * M_PI / n;
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      double acc = 0;
+      for (const Core& v : cores) {
+        const double dx = periodic_delta(c * h, v.x), dy = periodic_delta(r * h, v.y);
+        const double r2 = dx * dx + dy * dy, a2 = v.a * v.a;
+        if (r2 < 25 * a2) acc += v.amp * std::exp(-r2 / a2);
+      }
+      f[(size_t)r * n + c] = (float)acc;
+    }
+}
+
+static void build_scenario(int n, int scenario, double param, std::vector<float>& f) {
+  f.assign((size_t)n * n, 0.f);
+  const double h = 2 * M_PI / n;
+  if (scenario == S2T_SCENARIO_TAYLOR_GREEN) {
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c)
+        f[(size_t)r * n + c] = (float)(6 * std::cos(2 * c * h) * std::cos(2 * r * h));
+  } else if (scenario == S2T_SCENARIO_DIPOLE) {
+    const double a = 0.3, amp = 12 / (M_PI * a * a), x = 0.6 * M_PI;
+    gaussian_cores(n, {{x, M_PI - 0.45, a, amp}, {x, M_PI + 0.45, a, -amp}}, f);
+  } else if (scenario == S2T_SCENARIO_MERGER) {
+    const double d = param > 0 ? param : 0.7;
+    gaussian_cores(n, {{M_PI - d / 2, M_PI, 0.22, 30}, {M_PI + d / 2, M_PI, 0.22, 30}}, f);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// server state
+// ---------------------------------------------------------------------------------------------
+struct Request {
+  uint64_t id;
+  s2t_request req;
+  double t_submit;
+};
+
+struct Record {
+  int status = S2T_REQ_QUEUED;
+  int n = 0, steps = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  s2t_diag diag{};
+  std::vector<float> state;
+  bool keep_state = false;
+};
+
+struct Class {
+  Solver* s = nullptr;
+  std::vector<int64_t> slot_req;  // request id per slot, -1 when free
+  int active = 0;
+  double ms_per_step = 1.0;       // running estimate
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  size_t bytes = 0;
+};
+
+struct Server {
+  std::map<int, Class> cls;       // grid size -> class
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Request> pending;
+  std::unordered_map<uint64_t, Record> rec;
+  std::thread worker;
+  std::atomic<bool> stop{false};
+  uint64_t next_id = 1;
+  double target_ms = 4.0;         // GPU time per class per poll
+  size_t budget = 0, used = 0;
+  // statistics
+  std::vector<double> latencies;
+  double work_pts_steps = 0;      // sum n^2 * steps of completed requests
+  double t_first = 0, t_last = 0;
+  uint64_t submitted = 0, completed = 0, rejected = 0;
+  std::string fatal;
+};
+
+static size_t class_bytes(int n, int slots) {
+  // three states + mixed fields per slot, plus per-class scratch
+  const int K = n / 3, KC = K + 1, NJ = 2 * K + 1;
+  const size_t pitch = (NJ + 63) / 64 * 64;
+  const size_t state = (size_t)KC * pitch * 8, plane = (size_t)n * KC * 8;
+  return (size_t)slots * (3 * state + 5 * plane) + (size_t)n * n * 3 * 8 + state;
+}
+
+static void admit(Server* sv, Class& c, int slot, const Request& r) {
+  Solver* s = c.s;
+  const s2t_request& q = r.req;
+  configure_slot(s, slot, q.params);
+  if (q.scenario == S2T_SCENARIO_TURBULENCE) {
+    seed_spectrum_slot(s, slot, q.k0 > 0 ? q.k0 : 14.0, q.amplitude > 0 ? q.amplitude : 6.0,
+                       q.seed);
+  } else {
+    std::vector<float> f;
+    build_scenario(s->n, q.scenario, q.scenario_param, f);
+    seed_physical_slot(s, slot, f.data());
+  }
+  set_budget_slot(s, slot, q.steps);
+  c.slot_req[slot] = (int64_t)r.id;
+  c.active += 1;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  Record& rc = sv->rec[r.id];
+  rc.status = S2T_REQ_RUNNING;
+  rc.t_start = now_s();
+}
+
+static void complete(Server* sv, Class& c, int slot) {
+  const uint64_t id = (uint64_t)c.slot_req[slot];
+  s2t_diag d{};
+  diag_slot(c.s, slot, &d, nullptr, nullptr);
+  bool keep;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    keep = sv->rec[id].keep_state;
+  }
+  std::vector<float> st;
+  if (keep) {
+    st.resize((size_t)2 * c.s->n * c.s->n);
+    read_state_slot(c.s, slot, st.data());
+  }
+  const double t = now_s();
+  c.slot_req[slot] = -1;
+  c.active -= 1;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  Record& rc = sv->rec[id];
+  rc.status = S2T_REQ_DONE;
+  rc.t_done = t;
+  rc.diag = d;
+  rc.state.swap(st);
+  sv->latencies.push_back(t - rc.t_submit);
+  sv->work_pts_steps += (double)rc.n * rc.n * rc.steps;
+  sv->completed += 1;
+  sv->t_last = t;
+}
+
+static void worker_loop(Server* sv) {
+  try {
+    while (true) {
+      // 1. admission: FIFO per grid size, into free slots
+      std::vector<std::pair<int, Request>> take;
+      {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        bool any_active = false;
+        for (auto& kv : sv->cls) any_active |= kv.second.active > 0;
+        if (!any_active && sv->pending.empty()) {
+          if (sv->stop) break;
+          sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+          continue;
+        }
+        std::map<int, int> free_slots;
+        for (auto& kv : sv->cls) {
+          int f = 0;
+          for (int64_t r : kv.second.slot_req) f += r < 0;
+          free_slots[kv.first] = f;
+        }
+        for (auto it = sv->pending.begin(); it != sv->pending.end();) {
+          auto fs = free_slots.find(it->req.n);
+          if (fs != free_slots.end() && fs->second > 0) {
+            fs->second -= 1;
+            take.push_back({it->req.n, *it});
+            it = sv->pending.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      }
+      for (auto& tr : take) {
+        Class& c = sv->cls[tr.first];
+        int slot = 0;
+        while (c.slot_req[slot] >= 0) ++slot;
+        admit(sv, c, slot, tr.second);
+      }
+      // 2. step every class with active members, each on its own stream
+      std::vector<std::pair<Class*, int>> ran;
+      for (auto& kv : sv->cls) {
+        Class& c = kv.second;
+        if (c.active == 0) continue;
+        const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c.ms_per_step)));
+        ensure_primed(c.s);
+        hipchk(hipEventRecord(c.e0, c.s->stream), "hipEventRecord");
+        for (int i = 0; i < k; ++i) step_once(c.s, false);
+        hipchk(hipEventRecord(c.e1, c.s->stream), "hipEventRecord");
+        ran.push_back({&c, k});
+      }
+      // 3. poll: per-slot device flags tell which members retired
+      for (auto& pr : ran) {
+        Class& c = *pr.first;
+        sync(c.s);
+        float ms = 0.f;
+        hipchk(hipEventElapsedTime(&ms, c.e0, c.e1), "hipEventElapsedTime");
+        c.ms_per_step = 0.5 * c.ms_per_step + 0.5 * std::max(1e-3, (double)ms / pr.second);
+        for (int slot = 0; slot < c.s->batch; ++slot) {
+          if (c.slot_req[slot] < 0) continue;
+          const DevState d = read_clock_slot(c.s, slot);
+          if (!d.active) complete(sv, c, slot);
+        }
+      }
+    }
+  } catch (const std::exception& e) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->fatal = e.what();
+  }
+}
+
+static void server_free(Server* sv) {
+  if (!sv) return;
+  sv->stop = true;
+  sv->cv.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto& kv : sv->cls) {
+    if (kv.second.e0) (void)hipEventDestroy(kv.second.e0);
+    if (kv.second.e1) (void)hipEventDestroy(kv.second.e1);
+    solver_free(kv.second.s);
+  }
+  delete sv;
+}
+
+}  // namespace s2t
+
+using namespace s2t;
+
+#define S2T_API_BEGIN try {
+#define S2T_API_END                 \
+  return S2T_OK;                    \
+  }                                 \
+  catch (const Fail& f) {           \
+    g_err = f.what();               \
+    return f.code;                  \
+  }                                 \
+  catch (const std::exception& e) { \
+    g_err = e.what();               \
+    return S2T_EHIP;                \
+  }
+
+extern "C" {
+
+int s2t_server_create(const int* sizes, const int* slots, int nclasses, double budget_bytes,
+                      s2t_server** out) {
+  if (!out) return api_bad("s2t_server_create: out is NULL");
+  *out = nullptr;
+  if (!sizes || !slots || nclasses <= 0) return api_bad("s2t_server_create: no classes");
+  size_t need = 0;
+  for (int i = 0; i < nclasses; ++i) {
+    const std::string e = size_error(sizes[i]);
+    if (!e.empty()) return api_bad(e);
+    if (slots[i] <= 0) return api_bad("slot counts must be positive");
+    need += class_bytes(sizes[i], slots[i]);
+  }
+  if (budget_bytes > 0 && (double)need > budget_bytes)
+    return api_bad("slot capacity needs " + std::to_string(need) + " bytes, over the budget of " +
+                   std::to_string((size_t)budget_bytes));
+  Server* sv = new Server();
+  S2T_API_BEGIN
+  sv->budget = (size_t)budget_bytes;
+  for (int i = 0; i < nclasses; ++i) {
+    if (sv->cls.count(sizes[i])) throw Fail(S2T_EINVAL, "duplicate class size");
+    Class c;
+    c.s = solver_new(sizes[i], 1, slots[i], false);
+    c.slot_req.assign(slots[i], -1);
+    c.bytes = class_bytes(sizes[i], slots[i]);
+    hipchk(hipEventCreate(&c.e0), "hipEventCreate");
+    hipchk(hipEventCreate(&c.e1), "hipEventCreate");
+    // a new class starts with every slot idle
+    for (int k = 0; k < slots[i]; ++k) {
+      int zero = 0;
+      hipchk(hipMemcpy(&c.s->inst[k].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
+             "hipMemcpy");
+    }
+    sv->used += c.bytes;
+    sv->cls[sizes[i]] = c;
+  }
+  sv->worker = std::thread(worker_loop, sv);
+  *out = reinterpret_cast<s2t_server*>(sv);
+  return S2T_OK;
+  }
+  catch (const Fail& f) {
+    g_err = f.what();
+    server_free(sv);
+    return f.code;
+  }
+  catch (const std::exception& e) {
+    g_err = e.what();
+    server_free(sv);
+    return S2T_EHIP;
+  }
+}
+
+int s2t_server_destroy(s2t_server* h) {
+  server_free(reinterpret_cast<Server*>(h));
+  return S2T_OK;
+}
+
+int s2t_server_submit(s2t_server* h, const s2t_request* r, uint64_t* id) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv || !r || !id) return api_bad("s2t_server_submit: NULL argument");
+  if (r->steps <= 0) return api_bad("steps must be positive");
+  if (r->scenario < 0 || r->scenario > S2T_SCENARIO_MERGER) return api_bad("unknown scenario");
+  const std::string e = check_params(r->params);
+  if (!e.empty()) return api_bad(e);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!sv->fatal.empty()) return api_bad("server stopped: " + sv->fatal);
+  const uint64_t rid = sv->next_id++;
+  Record& rc = sv->rec[rid];
+  rc.n = r->n;
+  rc.steps = r->steps;
+  rc.t_submit = now_s();
+  rc.keep_state = r->keep_state != 0;
+  sv->submitted += 1;
+  if (sv->t_first == 0) sv->t_first = rc.t_submit;
+  if (!sv->cls.count(r->n)) {
+    rc.status = S2T_REQ_REJECTED;
+    sv->rejected += 1;
+  } else {
+    sv->pending.push_back({rid, *r, rc.t_submit});
+    sv->cv.notify_all();
+  }
+  *id = rid;
+  return S2T_OK;
+}
+
+int s2t_server_poll(s2t_server* h, uint64_t id, int* status, s2t_diag* diag, double* latency_s) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv || !status) return api_bad("s2t_server_poll: NULL argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!sv->fatal.empty()) return api_bad("server stopped: " + sv->fatal);
+  auto it = sv->rec.find(id);
+  if (it == sv->rec.end()) return api_bad("unknown request id");
+  *status = it->second.status;
+  if (diag) *diag = it->second.diag;
+  if (latency_s)
+    *latency_s = it->second.status == S2T_REQ_DONE ? it->second.t_done - it->second.t_submit : -1.0;
+  return S2T_OK;
+}
+
+int s2t_server_read_state(s2t_server* h, uint64_t id, float* spectrum) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv || !spectrum) return api_bad("s2t_server_read_state: NULL argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->rec.find(id);
+  if (it == sv->rec.end()) return api_bad("unknown request id");
+  if (it->second.status != S2T_REQ_DONE || it->second.state.empty())
+    return api_bad("request has no stored state (not done, or keep_state was 0)");
+  memcpy(spectrum, it->second.state.data(), it->second.state.size() * sizeof(float));
+  return S2T_OK;
+}
+
+int s2t_server_drain(s2t_server* h, double timeout_s) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv) return api_bad("s2t_server_drain: null server");
+  const double t0 = now_s();
+  while (true) {
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (!sv->fatal.empty()) return api_bad("server stopped: " + sv->fatal);
+      if (sv->completed + sv->rejected == sv->submitted) return S2T_OK;
+    }
+    if (timeout_s > 0 && now_s() - t0 > timeout_s) {
+      g_err = "drain timed out";
+      return S2T_ESTATE;
+    }
+    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+  }
+}
+
+int s2t_server_get_stats(s2t_server* h, s2t_server_stats* out) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv || !out) return api_bad("s2t_server_get_stats: NULL argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  std::vector<double> l = sv->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double p) {
+    if (l.empty()) return 0.0;
+    const double pos = p * (l.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - i;
+    return i + 1 < l.size() ? l[i] * (1 - f) + l[i + 1] * f : l[i];
+  };
+  out->submitted = sv->submitted;
+  out->completed = sv->completed;
+  out->rejected = sv->rejected;
+  out->p50_latency_s = pct(0.5);
+  out->p99_latency_s = pct(0.99);
+  out->mean_latency_s = l.empty() ? 0.0 : [&] { double t = 0; for (double x : l) t += x; return t / l.size(); }();
+  const double span = sv->t_last - sv->t_first;
+  out->mpts_steps_per_s = span > 0 ? sv->work_pts_steps / span / 1e6 : 0.0;
+  out->budget_bytes = (double)sv->budget;
+  out->used_bytes = (double)sv->used;
+  return S2T_OK;
+}
+
+}  // extern "C"

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/s2t.h"
 #include "physics.hpp"
@@ -20,7 +21,8 @@ struct Ops {
   // fused step kernels
   void (*rows)(Solver* s, int stage, bool speed, bool vel, bool force);
   void (*cols)(Solver* s, int stage, bool force);
-  void (*prime)(Solver* s);
+  void (*prime)(Solver* s, int i0, int ni);
+  void (*rhs)(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
   // full-field helpers
   void (*fft2)(Solver* s, float2* data, int inverse);
   void (*pack)(Solver* s, const float2* full, float2* state);
@@ -28,8 +30,6 @@ struct Ops {
 };
 
 const Ops* ops_for(int n);
-// fused right-hand side of the current mixed fields with explicit step index and dt (test hook)
-void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
 
 enum ColsMode { COLS_STEP = 0, COLS_PRIME = 1, COLS_RHS = 2 };
 
@@ -49,20 +49,21 @@ struct Solver {
   float2* full2 = nullptr;     // second full scratch (hooks), lazily
   float2* tw = nullptr;        // line-FFT twiddles
   double2* wtab = nullptr;     // exp(2 pi i m / N), FP64, for the zonal mean
-  DevState* dev = nullptr;
+  Inst* inst = nullptr;        // per-instance parameters and scalars (batch entries)
+  int batch = 1;               // instances sharing this solver's buffers
   double* diag_bins = nullptr; // [2][NBINS] energy and enstrophy per bin, then 2 totals
   double* diag_zonal = nullptr;
   double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation
   ushort4* dye[3] = {nullptr, nullptr, nullptr};
 
-  // host mirror of configuration
-  Phys phys{};
-  s2t_params params{};
-  bool configured = false;
-  Splats splats{};
-  int phase = 0;        // state[phase] holds the current spectrum
+  // host mirror of the configuration of each slot
+  std::vector<Phys> phys_slot;
+  std::vector<s2t_params> params_slot;
+  std::vector<char> primed_slot;  // mixed_u of the slot holds the prime of its state[phase]
+  Splats splats{};                // pointer splats (slot 0 of a plain solver)
+  int phase = 0;                  // state[phase] holds the current spectra of all slots
   int dye_parity = 0;
-  bool primed = false;  // mixed_u holds the prime of state[phase]
+  bool with_dye = false;
 
   // profiling
   bool prof = false;

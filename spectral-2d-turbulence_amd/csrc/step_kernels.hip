@@ -29,15 +29,18 @@
 
 namespace s2t {
 
-enum RowFlags { ROW_SPEED = 1, ROW_VEL = 2, ROW_FORCE = 4, ROW_CLOCK = 8 };
+enum RowFlags { ROW_SPEED = 1, ROW_VEL = 2, ROW_FORCE = 4 };
 
+// Buffers of instance i start at base + i * stride (blockIdx.y = i); `inst` holds the
+// per-instance parameters and scalars.
 struct RowArgs {
   const float2* __restrict__ mu;  // omega', u', v' tiles (F = 3)
   float2* __restrict__ mz;        // P', Q' tiles (F = 2)
   float2* __restrict__ mf;        // F' tiles (F = 1)
   float2* __restrict__ vel;       // physical (u, v), row-major N x N
   const float2* __restrict__ tw;
-  DevState* dev;
+  Inst* inst;
+  size_t s_mu, s_mz, s_mf, s_vel;  // per-instance strides (elements)
   Splats splats;
   int stage;
   int flags;
@@ -53,8 +56,8 @@ struct ColArgs {
   float2* __restrict__ fk;         // pointer-force spectrum (compact layout)
   float2* __restrict__ rhs_out;    // compact rhs (COLS_RHS)
   const float2* __restrict__ tw;
-  DevState* dev;
-  Phys ph;
+  Inst* inst;
+  size_t s_mz, s_mf, s_mu, s_state;  // per-instance strides (elements)
   int stage;
   int force;
   uint32_t step_override;
@@ -113,19 +116,17 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K;
   __shared__ float2 lds_all[G * L::LDS];
+  Inst* I = a.inst + blockIdx.y;
+  if (!I->st.active) return;  // uniform over the block
+  const size_t inst = blockIdx.y;
+  const float2* __restrict__ mu = a.mu + inst * a.s_mu;
+  float2* __restrict__ mz = a.mz + inst * a.s_mz;
+  float2* __restrict__ mf = (a.flags & ROW_FORCE) ? a.mf + inst * a.s_mf : a.mf;
+  float2* __restrict__ vel = (a.flags & ROW_VEL) ? a.vel + inst * a.s_vel : a.vel;
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
   const int yb = blockIdx.x * G + g;
   const bool live = yb < N / 4;
   float2* lds = lds_all + g * L::LDS;
-
-  if ((a.flags & ROW_CLOCK) && blockIdx.x == 0 && threadIdx.x == 0) {
-    if (a.stage == 0) {
-      a.dev->step += 1u;
-    } else if (a.stage == 1) {
-      a.dev->speed_bits = 0u;
-      a.dev->t += a.dev->dt;
-    }
-  }
 
   float smax = 0.f;
   bool bad = false;
@@ -140,17 +141,17 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
       float2 b = make_float2(0.f, 0.f);
       if (live) {
         if (k == 0) {
-          const float2 v0 = a.mu[Ge::template tile<3>(y0, 0, 2)];
-          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, 0, 2)];
+          const float2 v0 = mu[Ge::template tile<3>(y0, 0, 2)];
+          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, 0, 2)];
           b = make_float2(v0.x, v1.x);
         } else if (k <= K) {
-          const float2 v0 = a.mu[Ge::template tile<3>(y0, k, 2)];
-          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, k, 2)];
+          const float2 v0 = mu[Ge::template tile<3>(y0, k, 2)];
+          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, k, 2)];
           b = make_float2(v0.x - v1.y, v0.y + v1.x);
         } else if (k >= N - K) {
           const int m = N - k;
-          const float2 v0 = a.mu[Ge::template tile<3>(y0, m, 2)];
-          const float2 v1 = a.mu[Ge::template tile<3>(y0 + 1, m, 2)];
+          const float2 v0 = mu[Ge::template tile<3>(y0, m, 2)];
+          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, m, 2)];
           b = make_float2(v0.x + v1.y, v1.x - v0.y);
         }
       }
@@ -168,17 +169,17 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         float2 v = make_float2(0.f, 0.f);
         if (live) {
           if (k == 0) {
-            const float2 w = a.mu[Ge::template tile<3>(y, 0, 0)];
-            const float2 u = a.mu[Ge::template tile<3>(y, 0, 1)];
+            const float2 w = mu[Ge::template tile<3>(y, 0, 0)];
+            const float2 u = mu[Ge::template tile<3>(y, 0, 1)];
             v = make_float2(w.x, u.x);
           } else if (k <= K) {
-            const float2 w = a.mu[Ge::template tile<3>(y, k, 0)];
-            const float2 u = a.mu[Ge::template tile<3>(y, k, 1)];
+            const float2 w = mu[Ge::template tile<3>(y, k, 0)];
+            const float2 u = mu[Ge::template tile<3>(y, k, 1)];
             v = make_float2(w.x - u.y, w.y + u.x);
           } else if (k >= N - K) {
             const int m = N - k;
-            const float2 w = a.mu[Ge::template tile<3>(y, m, 0)];
-            const float2 u = a.mu[Ge::template tile<3>(y, m, 1)];
+            const float2 w = mu[Ge::template tile<3>(y, m, 0)];
+            const float2 u = mu[Ge::template tile<3>(y, m, 1)];
             v = make_float2(w.x + u.y, u.x - w.y);
           }
         }
@@ -195,11 +196,11 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
           bad |= !(sp <= 3.4e38f);
           smax = fmaxf(smax, sp);
         }
-        if ((a.flags & ROW_VEL) && live) a.vel[(size_t)y * N + t + q * T] = make_float2(u, v);
+        if ((a.flags & ROW_VEL) && live) vel[(unsigned)(y * N + t + q * T)] = make_float2(u, v);
         A[q] = make_float2(u * om, v * om);
       }
       line_fft<N, -1>(A, lds, t, a.tw);
-      row_unpack_store<N>(A, lds, t, live, a.mz, nullptr, y, 2);
+      row_unpack_store<N>(A, lds, t, live, mz, nullptr, y, 2);
     }
 
     if (a.flags & ROW_FORCE) {
@@ -213,15 +214,15 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
                     : make_float2(0.f, 0.f);
       }
       line_fft<N, -1>(B, lds, t, a.tw);
-      row_unpack_store<N>(B, lds, t, live, a.mf, a.mf, y0, 1);
+      row_unpack_store<N>(B, lds, t, live, mf, mf, y0, 1);
     }
   }
 
   if (a.flags & ROW_SPEED) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmaxf(smax, __shfl_xor(smax, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&a.dev->speed_bits, __float_as_uint(smax));
-    if (bad) atomicOr(&a.dev->nonfinite, 1u);
+    if ((threadIdx.x & 63) == 0) atomicMax(&I->st.speed_bits, __float_as_uint(smax));
+    if (bad) atomicOr(&I->st.nonfinite, 1u);
   }
 }
 
@@ -236,27 +237,43 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
   constexpr int GROUPS = (KC + C - 1) / C;
   constexpr int G8 = (GROUPS + 7) / 8 * 8;
+  constexpr int WSTRIDE = N + 2;
   __shared__ float2 lds_all[C * LSTRIDE];
+  // new state of the C columns, kept in LDS between the forward and inverse halves so the three
+  // inverse transforms do not hold it in registers
+  __shared__ float2 wbuf_all[C * WSTRIDE];
 
   // XCD-aware mapping: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD
   // a contiguous range of column groups (neighbouring tiles share L2 lines).
   const int bid = blockIdx.x;
   const int cg = (bid % 8) * (G8 / 8) + bid / 8;
   if (cg >= GROUPS) return;
+  Inst* I = a.inst + blockIdx.y;
+  if (!I->st.active) return;  // uniform over the block
+  const Phys& ph = I->ph;
+  const size_t inst = blockIdx.y;
+  const float2* __restrict__ mz = a.mz + inst * a.s_mz;
+  float2* __restrict__ mu = a.mu + inst * a.s_mu;
+  const float2* __restrict__ w0 = a.w0 + inst * a.s_state;
+  const float2* __restrict__ wcur = a.wcur + inst * a.s_state;
+  float2* __restrict__ wout = a.wout + inst * a.s_state;
+  const float2* __restrict__ mf = FORCE ? a.mf + inst * a.s_mf : a.mf;
+  float2* __restrict__ fk = FORCE ? a.fk + inst * a.s_state : a.fk;
+  float2* __restrict__ rhs_out = MODE == COLS_RHS ? a.rhs_out + inst * a.s_state : a.rhs_out;
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
   const int kx = cg * C + c;
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
+  float2* wbuf = wbuf_all + c * WSTRIDE;
   const float fkx = (float)kx;
-  const size_t col = (size_t)kx * Ge::PITCH;
+  const unsigned col = (unsigned)kx * Ge::PITCH;
 
-  float2 W[PTS];
   if constexpr (MODE == COLS_PRIME) {
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
       const int j = Ge::jrow(t + q * T);
-      W[q] = (active && j >= 0) ? a.wcur[col + j] : make_float2(0.f, 0.f);
+      wbuf[t + q * T] = (active && j >= 0) ? wcur[col + j] : make_float2(0.f, 0.f);
     }
   } else {
     double dt;
@@ -265,50 +282,63 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       dt = a.dt_override;
       step = a.step_override;
     } else {
-      step = a.dev->step;
+      step = I->st.step;
       if (a.stage == 0) {
-        const float speed = __uint_as_float(a.dev->speed_bits);
-        dt = a.ph.fixed_dt > 0.0 ? a.ph.fixed_dt : cfl_dt(speed, a.ph);
+        const float speed = __uint_as_float(I->st.speed_bits);
+        dt = ph.fixed_dt > 0.0 ? ph.fixed_dt : cfl_dt(speed, ph);
         if (bid == 0 && tid == 0) {
-          a.dev->dt = dt;
-          a.dev->speed = speed;
+          I->st.dt = dt;
+          I->st.speed = speed;
         }
       } else {
-        dt = a.dev->dt;
+        dt = I->st.dt;
       }
     }
-    float2 P[PTS], Q[PTS];
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int y = t + q * T;
-      P[q] = active ? a.mz[Ge::template tile<2>(y, kx, 0)] : make_float2(0.f, 0.f);
-      Q[q] = active ? a.mz[Ge::template tile<2>(y, kx, 1)] : make_float2(0.f, 0.f);
-    }
-    line_fft<N, -1>(P, lds, t, a.tw);
-    line_fft<N, -1>(Q, lds, t, a.tw);
-
-    float2 F[PTS];
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) F[q] = make_float2(0.f, 0.f);
+    // pointer-forcing spectrum: at stage 0 transform F' and keep the in-band modes in fk
     if constexpr (FORCE) {
       if (MODE == COLS_RHS || a.stage == 0) {
+        float2 F[PTS];
 #pragma unroll
         for (int q = 0; q < PTS; ++q) {
           const int y = t + q * T;
-          F[q] = active ? a.mf[Ge::template tile<1>(y, kx, 0)] : make_float2(0.f, 0.f);
+          F[q] = active ? mf[Ge::template tile<1>(y, kx, 0)] : make_float2(0.f, 0.f);
         }
         line_fft<N, -1>(F, lds, t, a.tw);
-      } else {
 #pragma unroll
         for (int q = 0; q < PTS; ++q) {
           const int j = Ge::jrow(t + q * T);
-          F[q] = (active && j >= 0) ? a.fk[col + j] : make_float2(0.f, 0.f);
+          if (active && j >= 0) fk[col + j] = F[q];
         }
       }
     }
+    // P_hat parks in wbuf and Q_hat in the exchange buffer, each thread at its own indices, so the
+    // per-mode loop below can run rolled with few live registers
+    {
+      float2 P[PTS];
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const int y = t + q * T;
+        P[q] = active ? mz[Ge::template tile<2>(y, kx, 0)] : make_float2(0.f, 0.f);
+      }
+      line_fft<N, -1>(P, lds, t, a.tw);
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) wbuf[t + q * T] = P[q];
+    }
+    {
+      float2 Q[PTS];
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const int y = t + q * T;
+        Q[q] = active ? mz[Ge::template tile<2>(y, kx, 1)] : make_float2(0.f, 0.f);
+      }
+      line_fft<N, -1>(Q, lds, t, a.tw);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) lds[pad(t + q * T)] = Q[q];
+    }
 
     const float fdt = (float)dt;
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < PTS; ++q) {
       const int idx = t + q * T;
       const int j = Ge::jrow(idx);
@@ -318,25 +348,25 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
         const float k2 = fkx * fkx + fky * fky;
         float2 rhs = make_float2(0.f, 0.f);
         if (k2 > 0.f) {
-          const float2 nh = nhat(P[q], Q[q], fkx, fky);
-          rhs = FORCE ? csub(F[q], nh) : make_float2(-nh.x, -nh.y);
-          if (a.ph.force_amp > 0.f)
-            rhs = cadd(rhs, stoch_force(idx, kx, fkx, fky, k2, a.ph, step, fdt, N));
+          const float2 nh = nhat(wbuf[idx], lds[pad(idx)], fkx, fky);
+          rhs = make_float2(-nh.x, -nh.y);
+          if constexpr (FORCE) rhs = cadd(rhs, fk[col + j]);
+          if (ph.force_amp > 0.f)
+            rhs = cadd(rhs, stoch_force(idx, kx, fkx, fky, k2, ph, step, fdt, N));
           if constexpr (MODE == COLS_STEP) {
-            const float2 w0 = a.w0[col + j];
-            const float2 wc = a.stage == 0 ? w0 : a.wcur[col + j];
-            const IFactor e = ifactor_half((double)kx, (double)k2, a.ph, dt);
-            wv = rk_update(a.stage, e, w0, wc, rhs, dt);
+            const float2 w0v = w0[col + j];
+            const float2 wc = a.stage == 0 ? w0v : wcur[col + j];
+            const IFactor e = ifactor_half((double)kx, (double)k2, ph, dt);
+            wv = rk_update(a.stage, e, w0v, wc, rhs, dt);
           }
         }
         if constexpr (MODE == COLS_RHS) {
-          a.rhs_out[col + j] = rhs;
+          rhs_out[col + j] = rhs;
         } else {
-          a.wout[col + j] = wv;
-          if (FORCE && a.stage == 0) a.fk[col + j] = F[q];
+          wout[col + j] = wv;
         }
       }
-      W[q] = wv;
+      wbuf[idx] = wv;
     }
     if constexpr (MODE == COLS_RHS) return;
   }
@@ -353,7 +383,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       const float fky = (float)wn(idx, N);
       const float k2 = fkx * fkx + fky * fky;
       const float inv = k2 > 0.f ? 1.f / k2 : 0.f;
-      const float2 w = cscale(W[q], S);
+      const float2 w = cscale(wbuf[idx], S);
       float2 uh, vh;
       vel_pack(w, fkx, fky, inv, uh, vh);
       X[q] = f == 0 ? w : (f == 1 ? uh : vh);
@@ -361,7 +391,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     line_fft<N, +1>(X, lds, t, a.tw);
     if (active) {
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) a.mu[Ge::template tile<3>(t + q * T, kx, f)] = X[q];
+      for (int q = 0; q < PTS; ++q) mu[Ge::template tile<3>(t + q * T, kx, f)] = X[q];
     }
   }
 }
@@ -473,20 +503,22 @@ struct Impl {
     a.mf = s->mixed_f;
     a.vel = s->vel;
     a.tw = s->tw;
-    a.dev = s->dev;
+    a.inst = s->inst;
+    a.s_mu = 3 * Ge::PLANE;
+    a.s_mz = 2 * Ge::PLANE;
+    a.s_mf = Ge::PLANE;
+    a.s_vel = (size_t)N * N;
     a.splats = s->splats;
     a.stage = stage;
-    a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0) |
-              (stage >= 0 ? ROW_CLOCK : 0);
-    if (stage < 0) a.stage = 1;
+    a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0);
     const int grid = (N / 4 + G - 1) / G;
     prof_begin(s, S2T_PROF_ROWS);
-    hipLaunchKernelGGL((k_rows<N, G>), dim3(grid), dim3(G * L::T), 0, s->stream, a);
+    hipLaunchKernelGGL((k_rows<N, G>), dim3(grid, s->batch), dim3(G * L::T), 0, s->stream, a);
     check_launch("k_rows");
     prof_end(s);
   }
 
-  static void cols(Solver* s, int stage, bool force) {
+  static ColArgs col_args(Solver* s, int stage) {
     ColArgs a{};
     a.mz = s->mixed_z;
     a.mf = s->mixed_f;
@@ -498,34 +530,61 @@ struct Impl {
     a.wout = s->state[(p + (stage == 1 ? 2 : 1)) % 3];
     a.fk = s->force_k;
     a.tw = s->tw;
-    a.dev = s->dev;
-    a.ph = s->phys;
+    a.inst = s->inst;
+    a.s_mz = 2 * Ge::PLANE;
+    a.s_mf = Ge::PLANE;
+    a.s_mu = 3 * Ge::PLANE;
+    a.s_state = Ge::STATE;
     a.stage = stage;
+    return a;
+  }
+
+  static void cols(Solver* s, int stage, bool force) {
+    ColArgs a = col_args(s, stage);
     a.force = force ? 1 : 0;
     constexpr int GROUPS = (Ge::KC + C - 1) / C;
     constexpr int G8 = (GROUPS + 7) / 8 * 8;
+    const dim3 grid(G8, s->batch);
     prof_begin(s, S2T_PROF_COLS);
     if (force)
-      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, true>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, true>), grid, dim3(C * L::T), 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, false>), grid, dim3(C * L::T), 0, s->stream, a);
     check_launch("k_cols");
     prof_end(s);
   }
 
-  static void prime(Solver* s) {
-    ColArgs a{};
-    a.mu = s->mixed_u;
-    a.wcur = s->state[s->phase];
-    a.tw = s->tw;
-    a.dev = s->dev;
-    a.ph = s->phys;
+  // velocity prime (encodePrime, solver.js:475-486) of instances i0 .. i0+ni-1 from state[phase]
+  static void prime(Solver* s, int i0, int ni) {
+    ColArgs a = col_args(s, 0);
+    a.wcur = s->state[s->phase] + (size_t)i0 * Ge::STATE;
+    a.mu = s->mixed_u + (size_t)i0 * 3 * Ge::PLANE;
+    a.inst = s->inst + i0;
     constexpr int GROUPS = (Ge::KC + C - 1) / C;
     constexpr int G8 = (GROUPS + 7) / 8 * 8;
     prof_begin(s, S2T_PROF_OTHER);
-    hipLaunchKernelGGL((k_cols<N, C, COLS_PRIME, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
+    hipLaunchKernelGGL((k_cols<N, C, COLS_PRIME, false>), dim3(G8, ni), dim3(C * L::T), 0,
+                       s->stream, a);
     check_launch("k_cols(prime)");
     prof_end(s);
+  }
+
+  // fused right-hand side of instance 0 with an explicit step index and dt (test hook)
+  static void rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
+    ColArgs a = col_args(s, 0);
+    a.rhs_out = rhs_out;
+    a.force = force ? 1 : 0;
+    a.step_override = step;
+    a.dt_override = dt;
+    constexpr int GROUPS = (Ge::KC + C - 1) / C;
+    constexpr int G8 = (GROUPS + 7) / 8 * 8;
+    if (force)
+      hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, true>), dim3(G8, 1), dim3(C * L::T), 0,
+                         s->stream, a);
+    else
+      hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, false>), dim3(G8, 1), dim3(C * L::T), 0,
+                         s->stream, a);
+    check_launch("k_cols(rhs)");
   }
 
   static void fft2(Solver* s, float2* data, int inverse) {
@@ -565,53 +624,10 @@ struct Impl {
 
   static const Ops* ops() {
     static const Ops o = {Ge::STATE, Ge::PLANE, L::TWSIZE, &twiddles, &rows, &cols, &prime,
-                          &fft2,     &pack,     &unpack};
+                          &rhs,      &fft2,     &pack,     &unpack};
     return &o;
   }
 };
-
-// RHS hook with explicit step index and dt (COLS_RHS reads them from the arguments)
-void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
-
-template <int N>
-static void cols_rhs_impl(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
-  using L = Line<N>;
-  using Ge = Geo<N>;
-  constexpr int C = ColCfg<N>::C;
-  ColArgs a{};
-  a.mz = s->mixed_z;
-  a.mf = s->mixed_f;
-  a.rhs_out = rhs_out;
-  a.tw = s->tw;
-  a.dev = s->dev;
-  a.ph = s->phys;
-  a.stage = 0;
-  a.force = force ? 1 : 0;
-  a.step_override = step;
-  a.dt_override = dt;
-  constexpr int GROUPS = (Ge::KC + C - 1) / C;
-  constexpr int G8 = (GROUPS + 7) / 8 * 8;
-  if (force)
-    hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, true>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
-  else
-    hipLaunchKernelGGL((k_cols<N, C, COLS_RHS, false>), dim3(G8), dim3(C * L::T), 0, s->stream, a);
-  check_launch("k_cols(rhs)");
-}
-
-void cols_rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
-  switch (s->n) {
-    case 16: return cols_rhs_impl<16>(s, force, rhs_out, step, dt);
-    case 32: return cols_rhs_impl<32>(s, force, rhs_out, step, dt);
-    case 64: return cols_rhs_impl<64>(s, force, rhs_out, step, dt);
-    case 128: return cols_rhs_impl<128>(s, force, rhs_out, step, dt);
-    case 256: return cols_rhs_impl<256>(s, force, rhs_out, step, dt);
-    case 512: return cols_rhs_impl<512>(s, force, rhs_out, step, dt);
-    case 1024: return cols_rhs_impl<1024>(s, force, rhs_out, step, dt);
-    case 2048: return cols_rhs_impl<2048>(s, force, rhs_out, step, dt);
-    case 4096: return cols_rhs_impl<4096>(s, force, rhs_out, step, dt);
-  }
-  throw std::runtime_error("unsupported grid size");
-}
 
 const Ops* ops_for(int n) {
   switch (n) {

@@ -191,7 +191,8 @@ def test_taylor_green_c1(s2t):
         w1 = s.read_state()
     ratio = w1[2, 2] / w0[2, 2]
     expect = np.exp(-8.0 * nu * 1.0)
-    assert abs(abs(ratio) / expect - 1) < 1e-6
+    # FP32 state rounded once per step (FP64 factors and combination): a random walk of ~1e-7
+    assert abs(abs(ratio) / expect - 1) < 5e-6
     # shape preserved: only the four (+-2, +-2) modes carry energy
     mask = np.zeros((n, n), bool)
     for r in (2, n - 2):

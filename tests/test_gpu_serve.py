@@ -1,0 +1,73 @@
+"""Continuous-batching server (s2t_server_*): results alone and in the mix are bit-identical;
+admission, rejection and latency statistics."""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PHYS = dict(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5, dt_max=0.02)
+
+
+def test_alone_equals_in_mix(s2t):
+    probe = dict(n=128, steps=40, scenario="merger", scenario_param=0.9, keep_state=True, **PHYS)
+    with s2t.Server({128: 2, 256: 2}) as sv:
+        rid = sv.submit(**probe)
+        sv.drain(120)
+        alone = sv.read_state(rid)
+        st_alone, d_alone, _ = sv.poll(rid)
+    with s2t.Server({64: 3, 128: 3, 256: 2}) as sv:
+        others = []
+        for k in range(6):
+            others.append(sv.submit(n=[64, 128, 256][k % 3], steps=15 + 10 * k,
+                                    scenario=["turbulence", "dipole", "taylor-green"][k % 3],
+                                    seed=k + 3, **PHYS))
+            if k == 2:
+                rid = sv.submit(**probe)
+            time.sleep(0.002)
+        sv.drain(120)
+        mixed = sv.read_state(rid)
+        st_mix, d_mix, _ = sv.poll(rid)
+        stats = sv.stats()
+    assert st_alone == st_mix == s2t.REQ_DONE
+    assert np.array_equal(alone, mixed)
+    assert d_alone["stepIndex"] == 40 and d_mix["stepIndex"] == 40
+    assert d_alone["energy"] == d_mix["energy"]
+    assert stats["completed"] == 7 and stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+
+
+def test_server_matches_plain_solver(s2t):
+    n, steps = 64, 25
+    with s2t.Server({n: 4}) as sv:
+        rid = sv.submit(n=n, steps=steps, scenario="turbulence", k0=6.0, amplitude=5.0, seed=11,
+                        keep_state=True, **PHYS)
+        sv.drain(60)
+        got = sv.read_state(rid)
+    with s2t.Solver(n) as s:
+        s.configure(**PHYS)
+        s.seed_spectrum(k0=6.0, amplitude=5.0, seed=11)
+        s.step(steps)
+        assert np.array_equal(got, s.read_state())
+
+
+def test_rejection_and_budget(s2t):
+    with s2t.Server({64: 2}) as sv:
+        rid = sv.submit(n=128, steps=5, **PHYS)
+        st, _, lat = sv.poll(rid)
+        assert st == s2t.REQ_REJECTED and lat < 0
+        sv.drain(10)
+        assert sv.stats()["rejected"] == 1
+    with pytest.raises(s2t.S2TError, match="over the budget"):
+        s2t.Server({2048: 8}, budget_bytes=1e6)
+
+
+def test_queueing_beyond_slots(s2t):
+    """More requests than slots wait for slots to free and all complete."""
+    with s2t.Server({64: 2}) as sv:
+        ids = [sv.submit(n=64, steps=10 + k, seed=k, **PHYS) for k in range(7)]
+        sv.drain(60)
+        for rid in ids:
+            st, d, lat = sv.poll(rid)
+            assert st == s2t.REQ_DONE and d["stepIndex"] >= 10 and lat > 0

@@ -86,6 +86,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: a single instance does not shard across GPUs "
@@ -93,6 +95,11 @@ def main(argv=None):
         return 2
 
     s2t = load_binding()
+    if args.copy_probe > 0:
+        ms = s2t.copy_probe(args.copy_probe, reps=5)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms,
+                          "GBps": 2 * args.copy_probe / ms / 1e6}))
+        return 0
     n = args.n
     params = workload_params(n)
     solver = s2t.Solver(n)

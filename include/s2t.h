@@ -275,6 +275,10 @@ int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);
 /* Algorithmic HBM bytes of one launch of a kernel kind for this solver's size and the current
  * options (stage 0 of a step when the kinds differ by stage). */
 int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes);
+/* Measurement helper with no counterpart in the reference: `reps` launches of a plain 16-byte
+ * streaming copy kernel (k_copy) of `bytes` bytes on a private stream; average ms per copy. Its
+ * counter readings calibrate the HBM traffic counters of the step kernels. */
+int s2t_copy_probe(double bytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ EXPORTS = [
     "s2t_read_state", "s2t_reset_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
-    "s2t_profile_read", "s2t_kernel_bytes",
+    "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
     "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
     "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
     "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
@@ -131,6 +131,7 @@ def load(path=None):
         "s2t_profile": (ip, [vp, ip]),
         "s2t_profile_read": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp)]),
         "s2t_kernel_bytes": (ip, [vp, ip, ip, ctypes.POINTER(dp)]),
+        "s2t_copy_probe": (ip, [dp, ip, ctypes.POINTER(dp)]),
         "s2t_batch_create": (ip, [ctypes.POINTER(ctypes.c_int), ip, ctypes.POINTER(vp)]),
         "s2t_batch_destroy": (ip, [vp]),
         "s2t_batch_size": (ip, [vp, ip, ctypes.POINTER(ip)]),
@@ -345,6 +346,14 @@ class Solver:
         b = ctypes.c_double()
         _check(self.lib, self.lib.s2t_kernel_bytes(self.h, kind, stage, ctypes.byref(b)))
         return b.value
+
+
+def copy_probe(nbytes, reps=10):
+    """Average ms of one plain device copy of nbytes (HBM counter calibration)."""
+    lib = load()
+    ms = ctypes.c_double()
+    _check(lib, lib.s2t_copy_probe(float(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def _params(kw):

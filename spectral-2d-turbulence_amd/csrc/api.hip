@@ -343,6 +343,40 @@ int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms) {
   S2T_API_END
 }
 
+int s2t_copy_probe(double bytes, int reps, double* ms_per_copy) {
+  if (!(bytes >= 16) || reps <= 0) return api_bad("s2t_copy_probe: bytes >= 16 and reps > 0");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = S2T_OK;
+  try {
+    const size_t n = (size_t)bytes / 16 * 16;
+    hipchk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    hipchk(hipMalloc(&a, n), "hipMalloc");
+    hipchk(hipMalloc(&b, n), "hipMalloc");
+    fill_u32(st, a, n / 4, 0x3f800000u);
+    hipchk(hipEventCreate(&e0), "hipEventCreate");
+    hipchk(hipEventCreate(&e1), "hipEventCreate");
+    copy_probe(st, a, b, n);  // warm-up
+    hipchk(hipEventRecord(e0, st), "hipEventRecord");
+    for (int i = 0; i < reps; ++i) copy_probe(st, a, b, n);
+    hipchk(hipEventRecord(e1, st), "hipEventRecord");
+    hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+    float ms = 0.f;
+    hipchk(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+  } catch (const Fail& f) {
+    g_err = f.what();
+    rc = f.code;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
 int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes) {
   SOLVER(h);
   if (!bytes) return api_bad("bytes is NULL");

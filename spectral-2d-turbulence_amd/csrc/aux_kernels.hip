@@ -83,6 +83,19 @@ void step_end(hipStream_t st, Inst* in, int ni) {
   check_launch("k_step_end");
 }
 
+// plain streaming copy (16-byte vectors, grid-stride): the reference kernel whose counter readings
+// calibrate the HBM traffic counters of the step kernels
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void copy_probe(hipStream_t st, const void* src, void* dst, size_t bytes) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
+  check_launch("k_copy");
+}
+
 // ---------------------------------------------------------------------------------------------
 // test hooks on full N x N fields
 // ---------------------------------------------------------------------------------------------

@@ -154,8 +154,8 @@ static void need_vel(Solver* s) {
   }
 }
 
-static float2* slot_state(Solver* s, int slot) {
-  return s->state[s->phase] + (size_t)slot * s->ops->state_elems;
+static float2* slot_state(Solver* s, int slot, int phase = -1) {
+  return s->state[phase >= 0 ? phase : s->phase] + (size_t)slot * s->ops->state_elems;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -285,16 +285,16 @@ void write_state_slot(Solver* s, int slot, const float* spectrum) {
   sync(s);
 }
 
-void read_state_slot(Solver* s, int slot, float* spectrum) {
+void read_state_slot(Solver* s, int slot, float* spectrum, int phase) {
   need_full(s);
-  s->ops->unpack(s, slot_state(s, slot), s->full);
+  s->ops->unpack(s, slot_state(s, slot, phase), s->full);
   d2h(s, spectrum, s->full, (size_t)s->n * s->n * sizeof(float2));
 }
 
 // readDiagnostics (solver.js:547-608) in FP64 with a fixed reduction order
-void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal) {
+void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal, int phase) {
   const int nb = s->n / 2 + 1;
-  diagnostics(s->stream, slot_state(s, slot), s->wtab, s->diag_bins, s->diag_zonal, s->n);
+  diagnostics(s->stream, slot_state(s, slot, phase), s->wtab, s->diag_bins, s->diag_zonal, s->n);
   std::vector<double> bins(2 * nb + 2);
   d2h(s, bins.data(), s->diag_bins, bins.size() * sizeof(double));
   const DevState d = read_clock_slot(s, slot);

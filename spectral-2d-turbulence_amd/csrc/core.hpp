@@ -32,8 +32,11 @@ void reset_slot(Solver* s, int slot);
 void seed_spectrum_slot(Solver* s, int slot, double k0, double amplitude, uint32_t seed);
 void seed_physical_slot(Solver* s, int slot, const float* field);
 void write_state_slot(Solver* s, int slot, const float* spectrum);
-void read_state_slot(Solver* s, int slot, float* spectrum);
-void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal);
+// phase >= 0 reads the slot from state[phase] instead of the solver's current phase (a slot that
+// retired early keeps its final state in the buffer of the step it retired on)
+void read_state_slot(Solver* s, int slot, float* spectrum, int phase = -1);
+void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zonal,
+               int phase = -1);
 // limit the slot to `steps` more steps, after which it retires itself (steps < 0: no limit)
 void set_budget_slot(Solver* s, int slot, int steps);
 DevState read_clock_slot(Solver* s, int slot);

@@ -27,16 +27,20 @@ struct Geo {
   static constexpr int NJ = 2 * K + 1;        // stored rows
   static constexpr int PITCH = (NJ + 63) / 64 * 64;
   static constexpr size_t STATE = (size_t)KC * PITCH;   // complex values per state field
-  static constexpr size_t PLANE = (size_t)N * KC;       // complex values per mixed field
+  static constexpr int KP = (KC + 15) / 16 * 16;        // mixed-field row pitch (128-byte lines)
+  static constexpr size_t PLANE = (size_t)N * KP;       // complex values per mixed field
   // compact row of row index i in [0, N); -1 when out of band
   static __device__ __forceinline__ int jrow(int i) {
     return i <= K ? i : (i >= N - K ? i - (N - NJ) : -1);
   }
-  // 32-bit element offsets (every buffer here holds fewer than 2^31 elements), so loads and
-  // stores take a uniform base plus one offset VGPR
+  // Mixed field f of a F-field buffer as row-major planes [f][y][kx]: the row kernel reads and
+  // writes whole rows as coalesced runs; the column kernel's column groups read 2-4 element runs
+  // per row that neighbouring groups on the same XCD share through L2. 32-bit element offsets
+  // (every buffer here holds fewer than 2^31 elements), so accesses take a uniform base plus one
+  // offset VGPR.
   template <int F>
   static __device__ __forceinline__ unsigned tile(int y, int kx, int f) {
-    return ((unsigned)((y >> 2) * KC + kx) * F + f) * 4u + (unsigned)(y & 3);
+    return (unsigned)f * (unsigned)PLANE + (unsigned)y * KP + (unsigned)kx;
   }
 };
 

@@ -103,6 +103,7 @@ struct Record {
 struct Class {
   Solver* s = nullptr;
   std::vector<int64_t> slot_req;  // request id per slot, -1 when free
+  std::vector<int> slot_phase;    // class buffer phase when the slot's request was admitted
   int active = 0;
   double ms_per_step = 1.0;       // running estimate
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -132,7 +133,8 @@ static size_t class_bytes(int n, int slots) {
   // three states + mixed fields per slot, plus per-class scratch
   const int K = n / 3, KC = K + 1, NJ = 2 * K + 1;
   const size_t pitch = (NJ + 63) / 64 * 64;
-  const size_t state = (size_t)KC * pitch * 8, plane = (size_t)n * KC * 8;
+  const size_t kp = (KC + 15) / 16 * 16;
+  const size_t state = (size_t)KC * pitch * 8, plane = (size_t)n * kp * 8;
   return (size_t)slots * (3 * state + 5 * plane) + (size_t)n * n * 3 * 8 + state;
 }
 
@@ -150,6 +152,7 @@ static void admit(Server* sv, Class& c, int slot, const Request& r) {
   }
   set_budget_slot(s, slot, q.steps);
   c.slot_req[slot] = (int64_t)r.id;
+  c.slot_phase[slot] = s->phase;
   c.active += 1;
   std::lock_guard<std::mutex> lk(sv->mu);
   Record& rc = sv->rec[r.id];
@@ -159,17 +162,22 @@ static void admit(Server* sv, Class& c, int slot, const Request& r) {
 
 static void complete(Server* sv, Class& c, int slot) {
   const uint64_t id = (uint64_t)c.slot_req[slot];
-  s2t_diag d{};
-  diag_slot(c.s, slot, &d, nullptr, nullptr);
   bool keep;
+  int steps;
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     keep = sv->rec[id].keep_state;
+    steps = sv->rec[id].steps;
   }
+  // the class kept rotating its buffers after this slot retired: its final state is where its
+  // own last step left it
+  const int phase = (c.slot_phase[slot] + steps) % 3;
+  s2t_diag d{};
+  diag_slot(c.s, slot, &d, nullptr, nullptr, phase);
   std::vector<float> st;
   if (keep) {
     st.resize((size_t)2 * c.s->n * c.s->n);
-    read_state_slot(c.s, slot, st.data());
+    read_state_slot(c.s, slot, st.data(), phase);
   }
   const double t = now_s();
   c.slot_req[slot] = -1;
@@ -310,6 +318,7 @@ int s2t_server_create(const int* sizes, const int* slots, int nclasses, double b
     Class c;
     c.s = solver_new(sizes[i], 1, slots[i], false);
     c.slot_req.assign(slots[i], -1);
+    c.slot_phase.assign(slots[i], 0);
     c.bytes = class_bytes(sizes[i], slots[i]);
     hipchk(hipEventCreate(&c.e0), "hipEventCreate");
     hipchk(hipEventCreate(&c.e1), "hipEventCreate");

@@ -19,10 +19,10 @@ def member_params(i):
 SIZES = [64, 128, 64, 256, 128, 64]
 
 
-def run_plain(s2t, n, params, seed, steps):
+def run_plain(s2t, n, params, seed, steps, k0=None):
     with s2t.Solver(n) as s:
         s.configure(**params)
-        s.seed_spectrum(k0=n / 10, amplitude=6.0, seed=seed)
+        s.seed_spectrum(k0=n / 10 if k0 is None else k0, amplitude=6.0, seed=seed)
         s.step(steps)
         return s.read_state(), s.diagnostics()
 
@@ -61,7 +61,7 @@ def test_batch_reseed_member_between_replays(s2t):
         b.step(6)
         got0 = b.read_state(0)
         got1 = b.read_state(1)
-    want0, _ = run_plain(s2t, 128, member_params(0), 7, 10)
+    want0, _ = run_plain(s2t, 128, member_params(0), 7, 10, k0=12)
     assert np.array_equal(got0, want0)
     with s2t.Solver(128) as s:
         s.configure(**member_params(1))

@@ -26,6 +26,27 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 B_CANONICAL = 208.0    # bytes/pt/step of the fused full-complex schedule used for the target
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at grid n from the committed rocprofv3 counter summary.
+
+    The summary (profiles/step_kernels_<n>_forced_turbulence.md) holds FETCH_SIZE and WRITE_SIZE
+    per dispatch of this workload, scaled by the copy-kernel calibration, as `TRAFFIC <name> <bytes>`
+    lines. The step variant without pointer forcing is the one the bench runs. None when absent.
+    """
+    path = os.path.join(ROOT, "profiles", "step_kernels_%d_forced_turbulence.md" % n)
+    if not os.path.exists(path):
+        return None
+    prefix = "s2t::%s<%d," % (kernel, n)
+    suffix = ",0,false>" if kernel == "k_cols" else ",false>"
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1].startswith(prefix) \
+                    and parts[1].endswith(suffix):
+                return float(parts[2])
+    return None
+
+
 def load_binding():
     spec = importlib.util.spec_from_file_location(
         "s2t_amd", os.path.join(ROOT, "spectral-2d-turbulence_amd", "__init__.py"))
@@ -156,7 +177,7 @@ def main(argv=None):
         "pct_hbm_roofline_208B": round(100.0 * B_CANONICAL * value * 1e6 / (HBM_PEAK_GBS * 1e9), 2),
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "traffic": measured_traffic(names[dom], n), "avg_launch_us": round(avg_s * 1e6, 2),
                      "bytes_per_launch": bytes_launch},
         "kernels": kernels,
     }

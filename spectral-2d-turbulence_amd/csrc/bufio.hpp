@@ -1,0 +1,49 @@
+// Raw buffer loads and stores for the step kernels (gfx9 buffer instructions).
+//
+// Each access is base (a 128-bit resource in SGPRs) + a 32-bit per-lane byte offset (VGPR) + a
+// 32-bit wave-uniform byte offset (SGPR). A line of PTS values per thread then costs one offset
+// VGPR in total: the per-value part of the address (q * T rows) is uniform and goes in the SGPR
+// operand. Flat 64-bit addressing would keep one VGPR pair per value live across a whole line
+// transform. Accesses at or beyond `bytes` of the resource read zero and drop the write.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace s2t {
+
+// word 3 of a raw (stride 0, untyped) buffer resource on gfx9
+constexpr int BUF_WORD3 = 0x00020000;
+
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+
+__device__ __forceinline__ Buf make_buf(const void* base, uint32_t bytes) {
+  return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                               BUF_WORD3)};
+}
+
+__device__ __forceinline__ float2 ld2(Buf b, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, (int)soff, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+
+__device__ __forceinline__ void st2(Buf b, float2 x, uint32_t voff, uint32_t soff) {
+  typedef unsigned u32x2 __attribute__((vector_size(8)));
+  u32x2 v;
+  v[0] = __float_as_uint(x.x);
+  v[1] = __float_as_uint(x.y);
+  __builtin_amdgcn_raw_buffer_store_b64(v, b.r, (int)voff, (int)soff, 0);
+}
+
+// Opaque copy of a value: loop-invariant code that depends on it cannot be hoisted out of the
+// loop it is laundered in (keeps per-line address and factor arrays from being held live across
+// line transforms).
+template <class T>
+__device__ __forceinline__ T launder(T x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+}  // namespace s2t

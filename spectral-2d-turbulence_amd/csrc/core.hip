@@ -94,10 +94,18 @@ s2t_params default_params() {
   return p;
 }
 
+// The reference's controls (src/main.js:150-190) offer dissipation orders 1, 2, 4, beta <= 80,
+// drag >= 0 and dtMax <= 0.05; the checks admit a superset that keeps the integrating factor's
+// exponent non-positive and its phase |beta kx / k^2 dt / 2| below 1e4.
 std::string check_params(const s2t_params& p) {
-  if (!(p.hyper_p > 0)) return "hyper_p must be positive";
+  if (!(p.hyper_p >= 1 && p.hyper_p <= 16 && std::floor(p.hyper_p) == p.hyper_p))
+    return "hyper_p must be an integer in 1..16";
+  if (!(p.drag >= 0)) return "drag must be non-negative";
+  if (!(p.viscosity >= 0)) return "viscosity must be non-negative";
   if (p.force_amp > 0 && !(p.force_width > 0)) return "force_width must be positive";
   if (!(p.dt_max > 0)) return "dt_max must be positive";
+  if (!(std::fabs(p.beta) * std::fmax(p.dt_max, p.fixed_dt) <= 1e4))
+    return "|beta| * max(dt_max, fixed_dt) must not exceed 1e4";
   return "";
 }
 
@@ -109,8 +117,7 @@ static Phys make_phys(int n, const s2t_params& p) {
   ph.mu = p.drag;
   ph.p = p.hyper_p;
   ph.nu = p.viscosity > 0 ? p.viscosity / std::pow(kcut, 2.0 * p.hyper_p) : 0.0;
-  const double ip = std::floor(p.hyper_p);
-  ph.p_int = (ip == p.hyper_p && ip >= 1 && ip <= 16) ? (int)ip : 0;
+  ph.p_int = (int)p.hyper_p;  // an integer in 1..16 (check_params)
   ph.force_amp = (float)p.force_amp;
   ph.force_k = (float)p.force_k;
   ph.force_width = (float)p.force_width;

@@ -157,11 +157,15 @@ struct Line {
   static constexpr int TW1 = 0;                              // pass-1 table offset
   static constexpr int TW2 = (R1 - 1) * R0;                  // pass-2 table offset
   static constexpr int TWSIZE = TW2 + (R2 > 1 ? (R2 - 1) * R0 * R1 : 0);
-  static constexpr int LDS = N + N / 16;                     // padded LDS floats2 per line
+  static constexpr int LDS = N + N / 32;                     // padded LDS float2 per line
 };
 
-// one padding slot every 16 elements breaks the stride-16 bank pattern of the first exchange
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+// LDS slot of line element i: one padding slot per 32 elements. The first exchange writes with
+// stride R0 = 16 and the later passes access runs of consecutive elements; with this padding a
+// 16- or 32-lane group of 64-bit accesses mostly lands on distinct bank pairs (32 or 64 banks),
+// and the slot stays affine in the element index, so a thread's accesses share one address
+// register and differ by immediate offsets.
+__device__ __forceinline__ int swz(int i) { return i + (i >> 5); }
 
 // Middle/last Stockham pass: NB butterflies of radix R per thread, span NS, inputs from LDS.
 template <int N, int R, int NS, int DIR, bool LAST>
@@ -176,7 +180,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
     const int b = t + i * T;
     const int k = b % NS;
 #pragma unroll
-    for (int r = 0; r < R; ++r) u[i][r] = lds[pad(b + r * STRIDE)];
+    for (int r = 0; r < R; ++r) u[i][r] = lds[swz(b + r * STRIDE)];
 #pragma unroll
     for (int r = 1; r < R; ++r) {
       float2 w = tw[(r - 1) * NS + k];
@@ -198,7 +202,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
       const int k = b % NS;
       const int base = (b / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[pad(base + r * NS)] = u[i][r];
+      for (int r = 0; r < R; ++r) lds[swz(base + r * NS)] = u[i][r];
     }
     __syncthreads();
   }
@@ -218,7 +222,7 @@ __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds,
   dft<L::R0, DIR>(v);
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < L::R0; ++r) lds[pad(t * L::R0 + r)] = v[r];
+  for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
   __syncthreads();
   if constexpr (L::NPASS == 2) {
     stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, twl + L::TW1);

@@ -52,7 +52,7 @@ __host__ __device__ __forceinline__ int wn(int i, int n) { return i < n / 2 ? i 
 struct Phys {
   double beta, mu, nu;   // nu = hyperviscosity coefficient nu_h
   double p;              // dissipation order
-  int p_int;             // p when it is a small positive integer, else 0
+  int p_int;             // p as an integer (1 .. 16)
   float force_amp, force_k, force_width, force_norm;
   double cfl, dt_min, dt_max, dx, fixed_dt;
   float dye_fade;
@@ -157,10 +157,13 @@ __device__ __forceinline__ double cfl_dt(float speed, const Phys& ph) {
 // still correctly rounded once to FP32. Method: Cody-Waite reduction by ln2 (resp. pi/2) with
 // split constants (high parts carry 32-33 fractional bits so n*hi is exact), then the Taylor
 // series of exp to r^12 on |r| <= ln2/2 and of sin / cos to r^15 / r^16 on |r| <= pi/4 (Horner).
-// Arguments outside the reduction's exact range fall back to the library routines.
+// exp_f64 is accurate over the whole double range (the 2^n scale is applied in two halves so
+// results up to the overflow threshold stay finite); sincos_f64 is accurate for |x| < 1e5 and
+// stays in [-1, 1] beyond (the solver's parameter checks keep the integrating factor's phase
+// below 1e4). No library fallback: its code would set the register budget of the step kernels.
 __host__ __device__ __forceinline__ double exp_f64(double y) {
   if (!(y > -745.0)) return y == y ? 0.0 : y;
-  if (!(y < 709.0)) return exp(y);
+  if (!(y < 710.0)) return y * HUGE_VAL;  // +inf, or NaN
   const double n = rint(y * 1.4426950408889634);
   double r = fma(-n, 0.6931471806019545, y);
   r = fma(-n, -4.2009150726810846e-11, r);
@@ -177,15 +180,11 @@ __host__ __device__ __forceinline__ double exp_f64(double y) {
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  return ldexp(p, (int)n);
+  const int ni = (int)n;
+  return ldexp(ldexp(p, ni >> 1), ni - (ni >> 1));
 }
 
 __host__ __device__ __forceinline__ void sincos_f64(double x, double* s, double* c) {
-  if (!(fabs(x) < 1.0e5)) {
-    *s = sin(x);
-    *c = cos(x);
-    return;
-  }
   const double q = rint(x * 0.6366197723675814);
   double r = fma(-q, 1.570796326850541, x);
   r = fma(-q, -5.5644316763895155e-11, r);
@@ -223,13 +222,9 @@ struct IFactor {
   double mag, c, s;  // E(dt/2) = mag * (c + i s)
 };
 __device__ __forceinline__ IFactor ifactor_half(double kx, double k2, const Phys& ph, double dt) {
-  double k2p;
-  if (ph.p_int > 0) {
-    k2p = k2;
-    for (int i = 1; i < ph.p_int; ++i) k2p *= k2;
-  } else {
-    k2p = pow(k2, ph.p);
-  }
+  // (k^2)^p for the integer order p >= 1 (the configuration checks reject other orders)
+  double k2p = k2;
+  for (int i = 1; i < ph.p_int; ++i) k2p *= k2;
   const double re = -ph.mu - ph.nu * k2p;
   IFactor e;
   e.mag = exp_f64(re * 0.5 * dt);

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <stdexcept>
 
+#include "bufio.hpp"
 #include "physics.hpp"
 #include "solver.hpp"
 
@@ -70,151 +71,171 @@ struct RowCfg {
 };
 template <int N>
 struct ColCfg {
-  // columns per workgroup: 4 (2 for the longest lines, whose five line buffers per thread need
-  // more than the 128 VGPRs of a 1024-thread workgroup)
+  // columns per workgroup: 2 for lines of 128+ threads (512-thread workgroups, two per CU at
+  // 64 KB of LDS each), 4 for 16..64 threads, else one 64-lane wave
   static constexpr int C = Line<N>::T >= 128 ? 2 : (Line<N>::T >= 16 ? 4 : 64 / Line<N>::T);
-  static constexpr int LSTRIDE = Line<N>::LDS + 2;                  // staggers column banks
+  // lanes of a wave take columns in runs of RUN consecutive lanes (one column per 16- or 32-lane
+  // LDS bank group for the long lines); the column buffers of the short lines are staggered
+  static constexpr int RUN = Line<N>::T < 32 ? Line<N>::T : 32;
+  static constexpr int LSTRIDE =
+      Line<N>::LDS + (N >= 512 ? 0 : N == 256 ? 2 : N >= 64 ? 8 : 4);
+  static __device__ __forceinline__ int col(int tid) { return (tid / RUN) % C; }
+  static __device__ __forceinline__ int row(int tid) { return (tid / (RUN * C)) * RUN + tid % RUN; }
 };
 
 // ---------------------------------------------------------------------------------------------
 // row kernel
 // ---------------------------------------------------------------------------------------------
+// Hermitian unpack along x of a forward row transform z held by the line (thread t holds
+// z[t + q T]): writes kx = 0..K of the two real fields packed as re / im. `voff` is the lane's
+// byte offset of (row, kx = t) in the destination plane; soff0 / soff1 are the uniform byte offsets
+// of the first / second field.
 template <int N>
 __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], float2* lds, int t,
-                                                 bool live, float2* __restrict__ out0,
-                                                 float2* __restrict__ out1, int y, int F) {
+                                                 bool live, Buf out, uint32_t voff,
+                                                 uint32_t soff0, uint32_t soff1) {
   using L = Line<N>;
   using Ge = Geo<N>;
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < L::PTS; ++q) lds[pad(t + q * L::T)] = z[q];
+  for (int q = 0; q < L::PTS; ++q) lds[swz(t + q * L::T)] = z[q];
   __syncthreads();
   if (!live) return;
 #pragma unroll
   for (int q = 0; q < L::PTS; ++q) {
     const int k = t + q * L::T;
-    if (k <= Ge::K) {
-      const float2 zk = lds[pad(k)];
-      const float2 zm = lds[pad((N - k) & (N - 1))];
+    if (q * L::T <= Ge::K && k <= Ge::K) {
+      const float2 zk = lds[swz(k)];
+      const float2 zm = lds[swz((N - k) & (N - 1))];
       float2 p, r;
       herm_unpack(zk, zm, p, r);
-      if (F == 2) {
-        out0[Ge::template tile<2>(y, k, 0)] = p;
-        out0[Ge::template tile<2>(y, k, 1)] = r;
-      } else {
-        // a pair of real rows packed as re/im: p is row y, r is row y + 1
-        out0[Ge::template tile<1>(y, k, 0)] = p;
-        out1[Ge::template tile<1>(y + 1, k, 0)] = r;
-      }
+      st2(out, p, voff, soff0 + (uint32_t)(q * L::T) * 8u);
+      st2(out, r, voff, soff1 + (uint32_t)(q * L::T) * 8u);
     }
   }
 }
 
-template <int N, int G>
-__global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
+// Inverse row transform input of the packed pair a + i b of two half spectra (planes fa, fb of
+// the mixed buffer, rows ya, yb), Hermitian-extended to the full row: thread t, value q holds
+// frequency k = t + q T. vlo / vhi are the lane's byte offsets of kx = t / kx = T - t in row 0;
+// soffa / soffb the uniform byte offsets of the two source rows.
+template <int N>
+__device__ __forceinline__ void row_load_pair(float2 (&x)[Line<N>::PTS], Buf mu, int t, bool live,
+                                              uint32_t vlo, uint32_t vhi, uint32_t soffa,
+                                              uint32_t soffb) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K;
+#pragma unroll
+  for (int q = 0; q < PTS; ++q) {
+    const int k = t + q * T;
+    float2 v = make_float2(0.f, 0.f);
+    if (live) {
+      if (2 * q < PTS) {
+        if (q * T <= K && k <= K) {
+          const uint32_t so = (uint32_t)(q * T) * 8u;
+          const float2 va = ld2(mu, vlo, soffa + so), vb = ld2(mu, vlo, soffb + so);
+          // k = 0: the real parts (the DC column of a real field)
+          v = k == 0 ? make_float2(va.x, vb.x) : make_float2(va.x - vb.y, va.y + vb.x);
+        }
+      } else if ((q + 1) * T - 1 >= N - K && k >= N - K) {
+        // m = N - k = (T - t) + (PTS - 1 - q) T
+        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * 8u;
+        const float2 va = ld2(mu, vhi, soffa + so), vb = ld2(mu, vhi, soffb + so);
+        v = make_float2(va.x + vb.y, vb.x - va.y);
+      }
+    }
+    x[q] = v;
+  }
+}
+
+template <int N, int G, bool FORCE>
+__global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int T = L::T, PTS = L::PTS;
+  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), ROW_B = (uint32_t)Ge::KP * 8;
   __shared__ float2 lds_all[G * L::LDS];
+  // v of the pair's second row, parked by each thread at its own indices while the first row is
+  // processed (holding it in registers instead would cost 16 VGPRs across two transforms)
+  __shared__ float vpark_all[G * N];
   Inst* I = a.inst + blockIdx.y;
   if (!I->st.active) return;  // uniform over the block
   const size_t inst = blockIdx.y;
-  const float2* __restrict__ mu = a.mu + inst * a.s_mu;
-  float2* __restrict__ mz = a.mz + inst * a.s_mz;
-  float2* __restrict__ mf = (a.flags & ROW_FORCE) ? a.mf + inst * a.s_mf : a.mf;
-  float2* __restrict__ vel = (a.flags & ROW_VEL) ? a.vel + inst * a.s_vel : a.vel;
+  constexpr bool do_force = FORCE;
+  const bool do_vel = a.flags & ROW_VEL;
+  const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
+  const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
+  const Buf bf = make_buf(do_force ? a.mf + inst * a.s_mf : a.mf, do_force ? PLANE_B : 0u);
+  const Buf bv = make_buf(do_vel ? a.vel + inst * a.s_vel : a.vel,
+                          do_vel ? (uint32_t)((size_t)N * N * 8) : 0u);
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
   const int yb = blockIdx.x * G + g;
   const bool live = yb < N / 4;
   float2* lds = lds_all + g * L::LDS;
+  float* vpark = vpark_all + g * N;
 
   float smax = 0.f;
   bool bad = false;
 #pragma unroll 1
   for (int pair = 0; pair < 2; ++pair) {
+    const int tp = launder(t);
     const int y0 = yb * 4 + pair * 2;
+    // lane byte offsets of kx = t and kx = T - t in row y0 of a plane
+    const uint32_t vlo = ((uint32_t)y0 * Ge::KP + (uint32_t)tp) * 8u;
+    const uint32_t vhi = ((uint32_t)y0 * Ge::KP + (uint32_t)(T - tp)) * 8u;
     // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
     float2 B[PTS];
+    row_load_pair<N>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 2 * PLANE_B + ROW_B);
+    line_fft<N, +1>(B, lds, tp, a.tw);
+    float v0[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
-      const int k = t + q * T;
-      float2 b = make_float2(0.f, 0.f);
-      if (live) {
-        if (k == 0) {
-          const float2 v0 = mu[Ge::template tile<3>(y0, 0, 2)];
-          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, 0, 2)];
-          b = make_float2(v0.x, v1.x);
-        } else if (k <= K) {
-          const float2 v0 = mu[Ge::template tile<3>(y0, k, 2)];
-          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, k, 2)];
-          b = make_float2(v0.x - v1.y, v0.y + v1.x);
-        } else if (k >= N - K) {
-          const int m = N - k;
-          const float2 v0 = mu[Ge::template tile<3>(y0, m, 2)];
-          const float2 v1 = mu[Ge::template tile<3>(y0 + 1, m, 2)];
-          b = make_float2(v0.x + v1.y, v1.x - v0.y);
-        }
-      }
-      B[q] = b;
+      v0[q] = B[q].x;
+      vpark[tp + q * T] = B[q].y;
     }
-    line_fft<N, +1>(B, lds, t, a.tw);
 
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {
+      const int tr = launder(tp);
       const int y = y0 + r;
+      const uint32_t vlr = ((uint32_t)y0 * Ge::KP + (uint32_t)tr) * 8u;
+      const uint32_t vhr = ((uint32_t)y0 * Ge::KP + (uint32_t)(T - tr)) * 8u;
+      // A' = omega' + i u' of row y
       float2 A[PTS];
-#pragma unroll
-      for (int q = 0; q < PTS; ++q) {
-        const int k = t + q * T;
-        float2 v = make_float2(0.f, 0.f);
-        if (live) {
-          if (k == 0) {
-            const float2 w = mu[Ge::template tile<3>(y, 0, 0)];
-            const float2 u = mu[Ge::template tile<3>(y, 0, 1)];
-            v = make_float2(w.x, u.x);
-          } else if (k <= K) {
-            const float2 w = mu[Ge::template tile<3>(y, k, 0)];
-            const float2 u = mu[Ge::template tile<3>(y, k, 1)];
-            v = make_float2(w.x - u.y, w.y + u.x);
-          } else if (k >= N - K) {
-            const int m = N - k;
-            const float2 w = mu[Ge::template tile<3>(y, m, 0)];
-            const float2 u = mu[Ge::template tile<3>(y, m, 1)];
-            v = make_float2(w.x + u.y, u.x - w.y);
-          }
-        }
-        A[q] = v;
-      }
-      line_fft<N, +1>(A, lds, t, a.tw);
+      row_load_pair<N>(A, bu, tr, live, vlr, vhr, (uint32_t)r * ROW_B, PLANE_B + (uint32_t)r * ROW_B);
+      line_fft<N, +1>(A, lds, tr, a.tw);
       // physical omega = Re A, u = Im A, v = Re/Im B; packed product Z = u omega + i v omega
+      const uint32_t vvel = ((uint32_t)y * N + (uint32_t)tr) * 8u;
 #pragma unroll
       for (int q = 0; q < PTS; ++q) {
         const float om = A[q].x, u = A[q].y;
-        const float v = r == 0 ? B[q].x : B[q].y;
+        const float v = r == 0 ? v0[q] : vpark[tr + q * T];
         if (a.flags & ROW_SPEED) {
           const float sp = fabsf(u) + fabsf(v);
           bad |= !(sp <= 3.4e38f);
           smax = fmaxf(smax, sp);
         }
-        if ((a.flags & ROW_VEL) && live) vel[(unsigned)(y * N + t + q * T)] = make_float2(u, v);
+        if (do_vel && live) st2(bv, make_float2(u, v), vvel, (uint32_t)(q * T) * 8u);
         A[q] = make_float2(u * om, v * om);
       }
-      line_fft<N, -1>(A, lds, t, a.tw);
-      row_unpack_store<N>(A, lds, t, live, mz, nullptr, y, 2);
+      line_fft<N, -1>(A, lds, tr, a.tw);
+      row_unpack_store<N>(A, lds, tr, live, bz, vlr, (uint32_t)r * ROW_B,
+                          PLANE_B + (uint32_t)r * ROW_B);
     }
 
-    if (a.flags & ROW_FORCE) {
+    if constexpr (FORCE) {
       // pointer forcing of rows y0, y0+1 at cell centres, packed as re/im of one transform
       const float yu0 = ((float)y0 + 0.5f) / (float)N;
       const float yu1 = ((float)(y0 + 1) + 0.5f) / (float)N;
 #pragma unroll
       for (int q = 0; q < PTS; ++q) {
-        const float xu = ((float)(t + q * T) + 0.5f) / (float)N;
+        const float xu = ((float)(tp + q * T) + 0.5f) / (float)N;
         B[q] = live ? make_float2(pointer_curl(xu, yu0, a.splats), pointer_curl(xu, yu1, a.splats))
                     : make_float2(0.f, 0.f);
       }
-      line_fft<N, -1>(B, lds, t, a.tw);
-      row_unpack_store<N>(B, lds, t, live, mf, mf, y0, 1);
+      line_fft<N, -1>(B, lds, tp, a.tw);
+      row_unpack_store<N>(B, lds, tp, live, bf, vlo, 0u, ROW_B);
     }
   }
 
@@ -233,18 +254,16 @@ template <int N, int C, int MODE, bool FORCE>
 __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   using L = Line<N>;
   using Ge = Geo<N>;
-  constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC;
+  constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC, NJ = Ge::NJ;
   constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
   constexpr int GROUPS = (KC + C - 1) / C;
   constexpr int G8 = (GROUPS + 7) / 8 * 8;
-  constexpr int WSTRIDE = N + 2;
+  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), STATE_B = (uint32_t)(Ge::STATE * 8);
+  constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;  // row block q -> q + 1 of a mixed plane
   __shared__ float2 lds_all[C * LSTRIDE];
-  // new state of the C columns, kept in LDS between the forward and inverse halves so the three
-  // inverse transforms do not hold it in registers
-  __shared__ float2 wbuf_all[C * WSTRIDE];
 
   // XCD-aware mapping: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD
-  // a contiguous range of column groups (neighbouring tiles share L2 lines).
+  // a contiguous range of column groups (neighbouring groups share L2 lines of the planes).
   const int bid = blockIdx.x;
   const int cg = (bid % 8) * (G8 / 8) + bid / 8;
   if (cg >= GROUPS) return;
@@ -252,30 +271,36 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   if (!I->st.active) return;  // uniform over the block
   const Phys& ph = I->ph;
   const size_t inst = blockIdx.y;
-  const float2* __restrict__ mz = a.mz + inst * a.s_mz;
-  float2* __restrict__ mu = a.mu + inst * a.s_mu;
-  const float2* __restrict__ w0 = a.w0 + inst * a.s_state;
-  const float2* __restrict__ wcur = a.wcur + inst * a.s_state;
-  float2* __restrict__ wout = a.wout + inst * a.s_state;
-  const float2* __restrict__ mf = FORCE ? a.mf + inst * a.s_mf : a.mf;
-  float2* __restrict__ fk = FORCE ? a.fk + inst * a.s_state : a.fk;
-  float2* __restrict__ rhs_out = MODE == COLS_RHS ? a.rhs_out + inst * a.s_state : a.rhs_out;
+  const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
+  const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
+  const Buf bw0 = make_buf(a.w0 + inst * a.s_state, STATE_B);
+  const Buf bwc = make_buf(a.wcur + inst * a.s_state, STATE_B);
+  // state leaving the stage (the right-hand side in COLS_RHS); P_hat is parked here first, each
+  // thread at the indices it later overwrites with the new state
+  const Buf bwo = MODE == COLS_RHS ? make_buf(a.rhs_out, STATE_B)
+                                   : make_buf(a.wout + inst * a.s_state, STATE_B);
+  const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
+  const Buf bfk = make_buf(FORCE ? a.fk + inst * a.s_state : a.fk, FORCE ? STATE_B : 0u);
   const int tid = threadIdx.x;
-  const int c = tid % C, t = tid / C;
+  const int c = ColCfg<N>::col(tid), t = ColCfg<N>::row(tid);
   const int kx = cg * C + c;
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
-  float2* wbuf = wbuf_all + c * WSTRIDE;
   const float fkx = (float)kx;
-  const unsigned col = (unsigned)kx * Ge::PITCH;
+  // per-lane byte offsets: row t of column kx in a mixed plane / in the compact state
+  const uint32_t vm = ((uint32_t)t * Ge::KP + (uint32_t)kx) * 8u;
+  const uint32_t vs = ((uint32_t)kx * Ge::PITCH + (uint32_t)t) * 8u;
+  // row block q holds rows t + q T: the low half are the compact rows j = i, the high half
+  // j = i - (N - NJ) (T divides N / 2, so a block never spans both halves)
+  auto soff_s = [](int q) -> uint32_t {
+    return (uint32_t)(q * T - (2 * q >= PTS ? N - NJ : 0)) * 8u;
+  };
+  auto inband = [&](int q) -> bool {
+    const int i = t + q * T;
+    return active && (2 * q < PTS ? i <= K : i >= N - K);
+  };
 
-  if constexpr (MODE == COLS_PRIME) {
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int j = Ge::jrow(t + q * T);
-      wbuf[t + q * T] = (active && j >= 0) ? wcur[col + j] : make_float2(0.f, 0.f);
-    }
-  } else {
+  if constexpr (MODE != COLS_PRIME) {
     double dt;
     uint32_t step;
     if constexpr (MODE == COLS_RHS) {
@@ -299,74 +324,77 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       if (MODE == COLS_RHS || a.stage == 0) {
         float2 F[PTS];
 #pragma unroll
-        for (int q = 0; q < PTS; ++q) {
-          const int y = t + q * T;
-          F[q] = active ? mf[Ge::template tile<1>(y, kx, 0)] : make_float2(0.f, 0.f);
-        }
+        for (int q = 0; q < PTS; ++q)
+          F[q] = active ? ld2(bf, vm, q * ROWS_B) : make_float2(0.f, 0.f);
         line_fft<N, -1>(F, lds, t, a.tw);
 #pragma unroll
-        for (int q = 0; q < PTS; ++q) {
-          const int j = Ge::jrow(t + q * T);
-          if (active && j >= 0) fk[col + j] = F[q];
-        }
+        for (int q = 0; q < PTS; ++q)
+          if (inband(q)) st2(bfk, F[q], vs, soff_s(q));
       }
     }
-    // P_hat parks in wbuf and Q_hat in the exchange buffer, each thread at its own indices, so the
-    // per-mode loop below can run rolled with few live registers
     {
       float2 P[PTS];
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) {
-        const int y = t + q * T;
-        P[q] = active ? mz[Ge::template tile<2>(y, kx, 0)] : make_float2(0.f, 0.f);
-      }
+      for (int q = 0; q < PTS; ++q)
+        P[q] = active ? ld2(bz, vm, q * ROWS_B) : make_float2(0.f, 0.f);
       line_fft<N, -1>(P, lds, t, a.tw);
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) wbuf[t + q * T] = P[q];
+      for (int q = 0; q < PTS; ++q)
+        if (inband(q)) st2(bwo, P[q], vs, soff_s(q));
     }
     {
+      // Q_hat parks in the exchange buffer, each thread at its own indices
       float2 Q[PTS];
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) {
-        const int y = t + q * T;
-        Q[q] = active ? mz[Ge::template tile<2>(y, kx, 1)] : make_float2(0.f, 0.f);
-      }
+      for (int q = 0; q < PTS; ++q)
+        Q[q] = active ? ld2(bz, vm, PLANE_B + q * ROWS_B) : make_float2(0.f, 0.f);
       line_fft<N, -1>(Q, lds, t, a.tw);
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) lds[pad(t + q * T)] = Q[q];
+      for (int q = 0; q < PTS; ++q) lds[swz(t + q * T)] = Q[q];
     }
 
+    // per-mode right-hand side and RK stage, one row block at a time (rolled, so only one
+    // block's values are live); the global operands of block q + 1 are fetched during block q
     const float fdt = (float)dt;
+    const bool need_wc = MODE == COLS_STEP && a.stage != 0;
+    const float2 zero = make_float2(0.f, 0.f);
+    float2 pn = zero, w0n = zero, wcn = zero, fkn = zero;
+    if (inband(0)) {
+      pn = ld2(bwo, vs, soff_s(0));
+      if (MODE == COLS_STEP) w0n = ld2(bw0, vs, soff_s(0));
+      if (need_wc) wcn = ld2(bwc, vs, soff_s(0));
+      if (FORCE) fkn = ld2(bfk, vs, soff_s(0));
+    }
 #pragma unroll 1
     for (int q = 0; q < PTS; ++q) {
-      const int idx = t + q * T;
-      const int j = Ge::jrow(idx);
-      float2 wv = make_float2(0.f, 0.f);
-      if (active && j >= 0) {
+      const float2 pv = pn, w0v = w0n, wcv = wcn, fkv = fkn;
+      const bool inb = inband(q);
+      if (q + 1 < PTS && inband(q + 1)) {
+        const uint32_t so = soff_s(q + 1);
+        pn = ld2(bwo, vs, so);
+        if (MODE == COLS_STEP) w0n = ld2(bw0, vs, so);
+        if (need_wc) wcn = ld2(bwc, vs, so);
+        if (FORCE) fkn = ld2(bfk, vs, so);
+      }
+      if (inb) {
+        const int idx = t + q * T;
         const float fky = (float)wn(idx, N);
         const float k2 = fkx * fkx + fky * fky;
-        float2 rhs = make_float2(0.f, 0.f);
+        float2 rhs = zero, wv = zero;
         if (k2 > 0.f) {
-          const float2 nh = nhat(wbuf[idx], lds[pad(idx)], fkx, fky);
+          const float2 nh = nhat(pv, lds[swz(idx)], fkx, fky);
           rhs = make_float2(-nh.x, -nh.y);
-          if constexpr (FORCE) rhs = cadd(rhs, fk[col + j]);
+          if constexpr (FORCE) rhs = cadd(rhs, fkv);
           if (ph.force_amp > 0.f)
             rhs = cadd(rhs, stoch_force(idx, kx, fkx, fky, k2, ph, step, fdt, N));
           if constexpr (MODE == COLS_STEP) {
-            const float2 w0v = w0[col + j];
-            const float2 wc = a.stage == 0 ? w0v : wcur[col + j];
             const IFactor e = ifactor_half((double)kx, (double)k2, ph, dt);
-            wv = rk_update(a.stage, e, w0v, wc, rhs, dt);
+            wv = rk_update(a.stage, e, w0v, a.stage == 0 ? w0v : wcv, rhs, dt);
           }
         }
-        if constexpr (MODE == COLS_RHS) {
-          rhs_out[col + j] = rhs;
-        } else {
-          wout[col + j] = wv;
-        }
+        st2(bwo, MODE == COLS_RHS ? rhs : wv, vs, soff_s(q));
       }
-      wbuf[idx] = wv;
     }
     if constexpr (MODE == COLS_RHS) return;
   }
@@ -374,24 +402,32 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   // velocity pack of the new state and inverse column transforms; the 1/N per inverse pass of
   // both directions (src/gpu/fft.js:59) is applied here as one exact 1/N^2
   constexpr float S = 1.0f / ((float)N * (float)N);
+  const Buf bsrc = MODE == COLS_PRIME ? bwc : bwo;
 #pragma unroll 1
   for (int f = 0; f < 3; ++f) {
+    // re-derive the lane's indices each field so nothing per-value stays live across transforms
+    const int tl = launder(t);
+    const int kxl = launder(kx);
+    const uint32_t vsl = ((uint32_t)kxl * Ge::PITCH + (uint32_t)tl) * 8u;
+    const uint32_t vml = ((uint32_t)tl * Ge::KP + (uint32_t)kxl) * 8u;
+    const float fkxl = (float)kxl;
     float2 X[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
-      const int idx = t + q * T;
+      const int idx = tl + q * T;
+      const bool inb = kxl < KC && (2 * q < PTS ? idx <= K : idx >= N - K);
+      const float2 w = cscale(inb ? ld2(bsrc, vsl, soff_s(q)) : make_float2(0.f, 0.f), S);
       const float fky = (float)wn(idx, N);
-      const float k2 = fkx * fkx + fky * fky;
+      const float k2 = fkxl * fkxl + fky * fky;
       const float inv = k2 > 0.f ? 1.f / k2 : 0.f;
-      const float2 w = cscale(wbuf[idx], S);
-      float2 uh, vh;
-      vel_pack(w, fkx, fky, inv, uh, vh);
-      X[q] = f == 0 ? w : (f == 1 ? uh : vh);
+      // u_hat = i ky / k^2 omega_hat, v_hat = -i kx / k^2 omega_hat
+      const float g = (f == 1 ? fky : -fkxl) * inv;
+      X[q] = f == 0 ? w : cscale(cmuli(w), g);
     }
-    line_fft<N, +1>(X, lds, t, a.tw);
-    if (active) {
+    line_fft<N, +1>(X, lds, tl, a.tw);
+    if (kxl < KC) {
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) mu[Ge::template tile<3>(t + q * T, kx, f)] = X[q];
+      for (int q = 0; q < PTS; ++q) st2(bu, X[q], vml, (uint32_t)f * PLANE_B + q * ROWS_B);
     }
   }
 }
@@ -425,7 +461,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_fft_cols(float2* data, const
   using L = Line<N>;
   constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
   __shared__ float2 lds_all[C * LSTRIDE];
-  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int c = ColCfg<N>::col(threadIdx.x), t = ColCfg<N>::row(threadIdx.x);
   const int colx = blockIdx.x * C + c;
   const bool live = colx < N;
   float2* lds = lds_all + c * LSTRIDE;
@@ -513,7 +549,12 @@ struct Impl {
     a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0);
     const int grid = (N / 4 + G - 1) / G;
     prof_begin(s, S2T_PROF_ROWS);
-    hipLaunchKernelGGL((k_rows<N, G>), dim3(grid, s->batch), dim3(G * L::T), 0, s->stream, a);
+    if (force)
+      hipLaunchKernelGGL((k_rows<N, G, true>), dim3(grid, s->batch), dim3(G * L::T), 0, s->stream,
+                         a);
+    else
+      hipLaunchKernelGGL((k_rows<N, G, false>), dim3(grid, s->batch), dim3(G * L::T), 0,
+                         s->stream, a);
     check_launch("k_rows");
     prof_end(s);
   }

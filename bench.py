@@ -10,6 +10,8 @@ describes the dominant kernel, timed with HIP events on the solver's stream duri
 steps; `cpu_baseline` times the FP64 NumPy oracle step on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-cpu-baseline]
+    python bench.py --workload ensemble [--steps K] [--plain-steps P]
+    python bench.py --workload serve [--rates r1,r2,...] [--requests R]
 """
 
 import argparse
@@ -99,6 +101,123 @@ def cpu_baseline(n, params, steps=1):
                        " CFL dt and stochastic forcing included, %.1f s" % (steps, n, el))
 
 
+ENSEMBLE_SIZES = [256] * 32 + [512] * 24 + [1024] * 8  # 16,777,216 points = 4096^2
+
+
+def ensemble_params(i, n):
+    """Member i of the mixed-size ensemble: Laplacian viscosity nu log-uniform in [1e-5, 1e-3]
+    (p = 1; the viscosity knob is the rate at the cut-off, nu * (n/3)^2), beta ~ U[0, 20],
+    forcing at k_f in {16, 32} +- 2 injecting 1e-3, CFL 0.5."""
+    rng = np.random.default_rng(1000 + i)
+    nu = 10 ** rng.uniform(-5, -3)
+    kf = float(rng.choice([16, 32]))
+    return dict(beta=float(rng.uniform(0, 20)), drag=0.0, viscosity=nu * (n / 3.0) ** 2,
+                hyper_p=1.0, force_amp=force_amp_for_rate(1e-3, n, kf, 2.0), force_k=kf,
+                force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+
+
+def bench_ensemble(s2t, steps, warmup, plain_steps):
+    """64 independent members (32 x 256^2, 24 x 512^2, 8 x 1024^2) advanced together by the
+    batch path (one launch per kernel and size class, hipGraph replay), against a loop over the
+    plain single-instance path on the same GPU."""
+    sizes = ENSEMBLE_SIZES
+    pts = sum(n * n for n in sizes)
+    with s2t.Batch(sizes) as b:
+        b.use_graphs(True)
+        for i, n in enumerate(sizes):
+            b.configure(i, **ensemble_params(i, n))
+            b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=1000 + i)
+        b.step(warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        el = time.perf_counter() - t0
+        bad = [i for i in range(len(sizes)) if not np.isfinite(b.diagnostics(i)["energy"])]
+    if bad:
+        raise RuntimeError("non-finite ensemble members %s" % bad)
+    batch_rate = pts * steps / el / 1e6
+    # plain path: each member on its own solver, stepped one after the other
+    el_plain = 0.0
+    for i, n in enumerate(sizes):
+        with s2t.Solver(n) as s:
+            s.configure(**ensemble_params(i, n))
+            s.seed_spectrum(k0=14.0, amplitude=6.0, seed=1000 + i)
+            s.step(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(plain_steps)
+            s.sync()
+            el_plain += time.perf_counter() - t0
+    plain_rate = pts * plain_steps / el_plain / 1e6
+    return {"metric": "ensemble Mpts·RK3-steps/s (64 members, 32x256^2 + 24x512^2 + 8x1024^2)",
+            "value": round(batch_rate, 2), "unit": "Mpts·steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(el * 1e3 / steps, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: forced turbulence members, seeds 1000+i",
+            "config": {"workload": "batched ensemble, 64 members, 16,777,216 points",
+                       "members": len(sizes), "points": pts},
+            "plain_loop": {"value": round(plain_rate, 2), "steps_per_member": plain_steps,
+                           "seconds": round(el_plain, 3)},
+            "speedup_vs_plain_loop": round(batch_rate / plain_rate, 3)}
+
+
+SERVE_SIZES = [128, 256, 512, 1024, 2048]
+SERVE_WEIGHTS = [0.3, 0.3, 0.2, 0.15, 0.05]
+SERVE_SLOTS = {128: 32, 256: 32, 512: 16, 1024: 8, 2048: 4}
+SERVE_SCENARIOS = ["taylor-green", "dipole", "merger", "turbulence"]
+
+
+def serve_requests(count, seed):
+    """Request mix: N by weight, steps log-uniform in [100, 5000], scenario uniform, merger
+    separation ~ U[0.5, 1.6]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice(SERVE_SIZES, p=SERVE_WEIGHTS))
+        steps = int(round(10 ** rng.uniform(2, np.log10(5000))))
+        sc = SERVE_SCENARIOS[int(rng.integers(4))]
+        param = float(rng.uniform(0.5, 1.6)) if sc == "merger" else 0.0
+        out.append(dict(n=n, steps=steps, scenario=sc, scenario_param=param, seed=k + 1))
+    return out
+
+
+def bench_serve(s2t, rates, count, seed=5):
+    """Poisson arrivals at each rate (requests/s) into the continuous-batching server; reports
+    completion-latency percentiles and the aggregate completed Mpts·steps/s."""
+    phys = dict(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5, dt_max=0.02)
+    reqs = serve_requests(count, seed)
+    work = float(np.mean([r["n"] ** 2 * r["steps"] for r in reqs]))
+    lines = []
+    for rate in rates:
+        rng = np.random.default_rng(seed + 1)
+        gaps = rng.exponential(1.0 / rate, size=count)
+        with s2t.Server(SERVE_SLOTS, budget_bytes=200e9) as sv:
+            t0 = time.perf_counter()
+            due = t0
+            for r, g in zip(reqs, gaps):
+                due += g
+                delay = due - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                sv.submit(**r, **phys)
+            sv.drain(600)
+            st = sv.stats()
+        lines.append({"rate_req_s": rate, "requests": count, "completed": st["completed"],
+                      "rejected": st["rejected"], "p50_latency_s": round(st["p50_latency_s"], 4),
+                      "p99_latency_s": round(st["p99_latency_s"], 4),
+                      "mean_latency_s": round(st["mean_latency_s"], 4),
+                      "mpts_steps_per_s": round(st["mpts_steps_per_s"], 1)})
+        print(json.dumps(lines[-1]), file=sys.stderr, flush=True)
+    return {"metric": "serving Mpts·RK3-steps/s under Poisson arrivals",
+            "value": max(l["mpts_steps_per_s"] for l in lines), "unit": "Mpts·steps/s",
+            "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: scenario mix, N in {128..2048}^2, steps log-uniform 100..5000",
+            "config": {"workload": "continuous-batching server", "slots": SERVE_SLOTS,
+                       "mean_work_pts_steps": work, "budget_bytes": 200e9},
+            "sweep": lines}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +226,14 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--workload", choices=["step", "ensemble", "serve"], default="step",
+                    help="step: one 4096^2 instance (default); ensemble: 64 mixed-size members; "
+                         "serve: Poisson arrivals into the serving loop")
+    ap.add_argument("--plain-steps", type=int, default=50,
+                    help="ensemble: steps per member of the plain-loop baseline")
+    ap.add_argument("--rates", type=str, default="",
+                    help="serve: comma-separated arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=120, help="serve: requests per rate")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args(argv)
@@ -120,6 +247,13 @@ def main(argv=None):
         ms = s2t.copy_probe(args.copy_probe, reps=5)
         print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms,
                           "GBps": 2 * args.copy_probe / ms / 1e6}))
+        return 0
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(s2t, args.steps, args.warmup, args.plain_steps)))
+        return 0
+    if args.workload == "serve":
+        rates = [float(r) for r in args.rates.split(",")] if args.rates else [5.0, 15.0, 30.0]
+        print(json.dumps(bench_serve(s2t, rates, args.requests)))
         return 0
     n = args.n
     params = workload_params(n)

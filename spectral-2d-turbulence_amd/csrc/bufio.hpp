@@ -37,6 +37,23 @@ __device__ __forceinline__ void st2(Buf b, float2 x, uint32_t voff, uint32_t sof
   __builtin_amdgcn_raw_buffer_store_b64(v, b.r, (int)voff, (int)soff, 0);
 }
 
+// 16-byte access: two adjacent complex values
+__device__ __forceinline__ void ld4(Buf b, uint32_t voff, uint32_t soff, float2& lo, float2& hi) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, (int)soff, 0);
+  lo = make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  hi = make_float2(__uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+__device__ __forceinline__ void st4(Buf b, float2 lo, float2 hi, uint32_t voff, uint32_t soff) {
+  typedef unsigned u32x4 __attribute__((vector_size(16)));
+  u32x4 v;
+  v[0] = __float_as_uint(lo.x);
+  v[1] = __float_as_uint(lo.y);
+  v[2] = __float_as_uint(hi.x);
+  v[3] = __float_as_uint(hi.y);
+  __builtin_amdgcn_raw_buffer_store_b128(v, b.r, (int)voff, (int)soff, 0);
+}
+
 // Opaque copy of a value: loop-invariant code that depends on it cannot be hoisted out of the
 // loop it is laundered in (keeps per-line address and factor arrays from being held live across
 // line transforms).

@@ -33,14 +33,15 @@ struct Geo {
   static __device__ __forceinline__ int jrow(int i) {
     return i <= K ? i : (i >= N - K ? i - (N - NJ) : -1);
   }
-  // Mixed field f of a F-field buffer as row-major planes [f][y][kx]: the row kernel reads and
-  // writes whole rows as coalesced runs; the column kernel's column groups read 2-4 element runs
-  // per row that neighbouring groups on the same XCD share through L2. 32-bit element offsets
-  // (every buffer here holds fewer than 2^31 elements), so accesses take a uniform base plus one
-  // offset VGPR.
+  // Mixed field f of an F-field buffer as planes of 2-row tiles [f][y/2][kx][2]: the values of
+  // rows 2m and 2m+1 at one kx are adjacent (16 bytes). The row kernel, which works on row pairs,
+  // reads a pair of rows as one contiguous run of 16-byte elements; a column group of the column
+  // kernel touches half as many cache lines per access as with row-major planes. 32-bit element
+  // offsets (every buffer here holds fewer than 2^31 elements).
   template <int F>
   static __device__ __forceinline__ unsigned tile(int y, int kx, int f) {
-    return (unsigned)f * (unsigned)PLANE + (unsigned)y * KP + (unsigned)kx;
+    return (unsigned)f * (unsigned)PLANE + ((unsigned)(y >> 1) * KP + (unsigned)kx) * 2u +
+           (unsigned)(y & 1);
   }
 };
 

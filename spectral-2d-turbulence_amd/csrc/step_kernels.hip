@@ -87,10 +87,11 @@ struct ColCfg {
 // row kernel
 // ---------------------------------------------------------------------------------------------
 // Hermitian unpack along x of a forward row transform z held by the line (thread t holds
-// z[t + q T]): writes kx = 0..K of the two real fields packed as re / im. `voff` is the lane's
-// byte offset of (row, kx = t) in the destination plane; soff0 / soff1 are the uniform byte offsets
-// of the first / second field.
-template <int N>
+// z[t + q T]): the two real fields packed as re / im, for kx = 0..K, into 2-row-tiled planes.
+// `voff` is the lane's byte offset of (row pair, kx = t). PAIR: the fields are rows 2m and 2m+1
+// of one plane, stored together as one 16-byte element; otherwise they are one row of two planes
+// (uniform byte offsets soff0 / soff1, which include the row's 8-byte slot in the pair).
+template <int N, bool PAIR>
 __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], float2* lds, int t,
                                                  bool live, Buf out, uint32_t voff,
                                                  uint32_t soff0, uint32_t soff1) {
@@ -109,20 +110,26 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
       const float2 zm = lds[swz((N - k) & (N - 1))];
       float2 p, r;
       herm_unpack(zk, zm, p, r);
-      st2(out, p, voff, soff0 + (uint32_t)(q * L::T) * 8u);
-      st2(out, r, voff, soff1 + (uint32_t)(q * L::T) * 8u);
+      const uint32_t so = (uint32_t)(q * L::T) * 16u;
+      if constexpr (PAIR) {
+        st4(out, p, r, voff, soff0 + so);
+      } else {
+        st2(out, p, voff, soff0 + so);
+        st2(out, r, voff, soff1 + so);
+      }
     }
   }
 }
 
-// Inverse row transform input of the packed pair a + i b of two half spectra (planes fa, fb of
-// the mixed buffer, rows ya, yb), Hermitian-extended to the full row: thread t, value q holds
-// frequency k = t + q T. vlo / vhi are the lane's byte offsets of kx = t / kx = T - t in row 0;
-// soffa / soffb the uniform byte offsets of the two source rows.
-template <int N>
-__device__ __forceinline__ void row_load_pair(float2 (&x)[Line<N>::PTS], Buf mu, int t, bool live,
-                                              uint32_t vlo, uint32_t vhi, uint32_t soffa,
-                                              uint32_t soffb) {
+// Inverse row transform input, Hermitian-extended to the full row: thread t, value q holds
+// frequency k = t + q T. vlo / vhi are the lane's byte offsets of (row pair, kx = t) and
+// (row pair, kx = T - t) in a 2-row-tiled plane. VPAIR: x = a(y0) + i a(y0+1) for one real field
+// a (plane offset soffa; one 16-byte load per value); otherwise x = a(y) + i b(y) for two real
+// fields of one row (uniform offsets soffa / soffb include the row's 8-byte slot).
+template <int N, bool VPAIR>
+__device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int t, bool live,
+                                         uint32_t vlo, uint32_t vhi, uint32_t soffa,
+                                         uint32_t soffb) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K;
@@ -133,15 +140,27 @@ __device__ __forceinline__ void row_load_pair(float2 (&x)[Line<N>::PTS], Buf mu,
     if (live) {
       if (2 * q < PTS) {
         if (q * T <= K && k <= K) {
-          const uint32_t so = (uint32_t)(q * T) * 8u;
-          const float2 va = ld2(mu, vlo, soffa + so), vb = ld2(mu, vlo, soffb + so);
+          const uint32_t so = (uint32_t)(q * T) * 16u;
+          float2 va, vb;
+          if constexpr (VPAIR) {
+            ld4(mu, vlo, soffa + so, va, vb);
+          } else {
+            va = ld2(mu, vlo, soffa + so);
+            vb = ld2(mu, vlo, soffb + so);
+          }
           // k = 0: the real parts (the DC column of a real field)
           v = k == 0 ? make_float2(va.x, vb.x) : make_float2(va.x - vb.y, va.y + vb.x);
         }
       } else if ((q + 1) * T - 1 >= N - K && k >= N - K) {
         // m = N - k = (T - t) + (PTS - 1 - q) T
-        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * 8u;
-        const float2 va = ld2(mu, vhi, soffa + so), vb = ld2(mu, vhi, soffb + so);
+        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * 16u;
+        float2 va, vb;
+        if constexpr (VPAIR) {
+          ld4(mu, vhi, soffa + so, va, vb);
+        } else {
+          va = ld2(mu, vhi, soffa + so);
+          vb = ld2(mu, vhi, soffb + so);
+        }
         v = make_float2(va.x + vb.y, vb.x - va.y);
       }
     }
@@ -154,7 +173,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS;
-  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), ROW_B = (uint32_t)Ge::KP * 8;
+  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8);
   __shared__ float2 lds_all[G * L::LDS];
   // v of the pair's second row, parked by each thread at its own indices while the first row is
   // processed (holding it in registers instead would cost 16 VGPRs across two transforms)
@@ -181,12 +200,12 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   for (int pair = 0; pair < 2; ++pair) {
     const int tp = launder(t);
     const int y0 = yb * 4 + pair * 2;
-    // lane byte offsets of kx = t and kx = T - t in row y0 of a plane
-    const uint32_t vlo = ((uint32_t)y0 * Ge::KP + (uint32_t)tp) * 8u;
-    const uint32_t vhi = ((uint32_t)y0 * Ge::KP + (uint32_t)(T - tp)) * 8u;
+    // lane byte offsets of (row pair y0 / 2, kx = t) and (row pair, kx = T - t) in a plane
+    const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
+    const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
     // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
     float2 B[PTS];
-    row_load_pair<N>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 2 * PLANE_B + ROW_B);
+    row_load<N, true>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 0u);
     line_fft<N, +1>(B, lds, tp, a.tw);
     float v0[PTS];
 #pragma unroll
@@ -199,11 +218,11 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
     for (int r = 0; r < 2; ++r) {
       const int tr = launder(tp);
       const int y = y0 + r;
-      const uint32_t vlr = ((uint32_t)y0 * Ge::KP + (uint32_t)tr) * 8u;
-      const uint32_t vhr = ((uint32_t)y0 * Ge::KP + (uint32_t)(T - tr)) * 8u;
-      // A' = omega' + i u' of row y
+      const uint32_t vlr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tr) * 16u;
+      const uint32_t vhr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tr)) * 16u;
+      // A' = omega' + i u' of row y (its 8-byte slot r in the row pair)
       float2 A[PTS];
-      row_load_pair<N>(A, bu, tr, live, vlr, vhr, (uint32_t)r * ROW_B, PLANE_B + (uint32_t)r * ROW_B);
+      row_load<N, false>(A, bu, tr, live, vlr, vhr, (uint32_t)r * 8u, PLANE_B + (uint32_t)r * 8u);
       line_fft<N, +1>(A, lds, tr, a.tw);
       // physical omega = Re A, u = Im A, v = Re/Im B; packed product Z = u omega + i v omega
       const uint32_t vvel = ((uint32_t)y * N + (uint32_t)tr) * 8u;
@@ -220,8 +239,8 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         A[q] = make_float2(u * om, v * om);
       }
       line_fft<N, -1>(A, lds, tr, a.tw);
-      row_unpack_store<N>(A, lds, tr, live, bz, vlr, (uint32_t)r * ROW_B,
-                          PLANE_B + (uint32_t)r * ROW_B);
+      row_unpack_store<N, false>(A, lds, tr, live, bz, vlr, (uint32_t)r * 8u,
+                                 PLANE_B + (uint32_t)r * 8u);
     }
 
     if constexpr (FORCE) {
@@ -235,7 +254,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
                     : make_float2(0.f, 0.f);
       }
       line_fft<N, -1>(B, lds, tp, a.tw);
-      row_unpack_store<N>(B, lds, tp, live, bf, vlo, 0u, ROW_B);
+      row_unpack_store<N, true>(B, lds, tp, live, bf, vlo, 0u, 0u);
     }
   }
 
@@ -259,7 +278,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   constexpr int GROUPS = (KC + C - 1) / C;
   constexpr int G8 = (GROUPS + 7) / 8 * 8;
   constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), STATE_B = (uint32_t)(Ge::STATE * 8);
-  constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;  // row block q -> q + 1 of a mixed plane
+  constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;  // row block q -> q + 1 (T / 2 row pairs)
   __shared__ float2 lds_all[C * LSTRIDE];
 
   // XCD-aware mapping: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD
@@ -287,8 +306,9 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
   const float fkx = (float)kx;
-  // per-lane byte offsets: row t of column kx in a mixed plane / in the compact state
-  const uint32_t vm = ((uint32_t)t * Ge::KP + (uint32_t)kx) * 8u;
+  // per-lane byte offsets: row t of column kx in a 2-row-tiled mixed plane / in the compact state
+  // (row block q adds q T rows: q T / 2 row pairs, T is even)
+  const uint32_t vm = (((uint32_t)(t >> 1) * Ge::KP + (uint32_t)kx) * 2u + (uint32_t)(t & 1)) * 8u;
   const uint32_t vs = ((uint32_t)kx * Ge::PITCH + (uint32_t)t) * 8u;
   // row block q holds rows t + q T: the low half are the compact rows j = i, the high half
   // j = i - (N - NJ) (T divides N / 2, so a block never spans both halves)
@@ -409,7 +429,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     const int tl = launder(t);
     const int kxl = launder(kx);
     const uint32_t vsl = ((uint32_t)kxl * Ge::PITCH + (uint32_t)tl) * 8u;
-    const uint32_t vml = ((uint32_t)tl * Ge::KP + (uint32_t)kxl) * 8u;
+    const uint32_t vml =
+        (((uint32_t)(tl >> 1) * Ge::KP + (uint32_t)kxl) * 2u + (uint32_t)(tl & 1)) * 8u;
     const float fkxl = (float)kxl;
     float2 X[PTS];
 #pragma unroll

@@ -122,6 +122,11 @@ static Phys make_phys(int n, const s2t_params& p) {
   ph.force_k = (float)p.force_k;
   ph.force_width = (float)p.force_width;
   ph.force_norm = (float)(0.01 * n * n);  // src/gpu/shaders.js:26
+  {
+    const double lo = p.force_k - 3.05 * p.force_width, hi = p.force_k + 3.05 * p.force_width;
+    ph.force_k2lo = lo > 0 ? (float)(lo * lo * (1.0 - 1e-5)) : -1.0f;
+    ph.force_k2hi = (float)(hi * hi * (1.0 + 1e-5));
+  }
   ph.cfl = p.cfl;
   ph.dt_min = p.dt_min;
   ph.dt_max = p.dt_max;

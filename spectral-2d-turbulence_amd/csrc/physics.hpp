@@ -55,6 +55,9 @@ struct Phys {
   double p;              // dissipation order
   int p_int;             // p as an integer (1 .. 16)
   float force_amp, force_k, force_width, force_norm;
+  // |k|^2 outside [force_k2lo, force_k2hi] has envelope exp(-x^2) < 1e-4 by a margin
+  // (|x| >= 3.05 > sqrt(ln 1e4) = 3.035): skipped without evaluating it
+  float force_k2lo, force_k2hi;
   double cfl, dt_min, dt_max, dx, fixed_dt;
   float dye_fade;
 };
@@ -130,6 +133,7 @@ __device__ __forceinline__ float2 nhat(float2 q1, float2 q2, float kx, float ky)
 // grid indices of the mode; the canonical half is ky > 0, or ky = 0 and kx > 0.
 __device__ __forceinline__ float2 stoch_force(int row, int col, float kx, float ky, float k2,
                                               const Phys& ph, uint32_t step, float dt, int n) {
+  if (k2 < ph.force_k2lo || k2 > ph.force_k2hi) return make_float2(0.f, 0.f);
   const float x = (sqrtf(k2) - ph.force_k) / ph.force_width;
   const float env = expf(-(x * x));
   if (!(env > 1e-4f)) return make_float2(0.f, 0.f);

@@ -390,6 +390,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     for (int q = 0; q < PTS; ++q) {
       const float2 pv = pn, w0v = w0n, wcv = wcn, fkv = fkn;
       const bool inb = inband(q);
+      float2 wv = zero;
       if (q + 1 < PTS && inband(q + 1)) {
         const uint32_t so = soff_s(q + 1);
         pn = ld2(bwo, vs, so);
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
         const int idx = t + q * T;
         const float fky = (float)wn(idx, N);
         const float k2 = fkx * fkx + fky * fky;
-        float2 rhs = zero, wv = zero;
+        float2 rhs = zero;
         if (k2 > 0.f) {
           const float2 nh = nhat(pv, lds[swz(idx)], fkx, fky);
           rhs = make_float2(-nh.x, -nh.y);
@@ -415,6 +416,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
         }
         st2(bwo, MODE == COLS_RHS ? rhs : wv, vs, soff_s(q));
       }
+      // the new state replaces Q_hat in this thread's own slot of the exchange buffer
+      if constexpr (MODE == COLS_STEP) lds[swz(t + q * T)] = wv;
     }
     if constexpr (MODE == COLS_RHS) return;
   }
@@ -422,13 +425,22 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   // velocity pack of the new state and inverse column transforms; the 1/N per inverse pass of
   // both directions (src/gpu/fft.js:59) is applied here as one exact 1/N^2
   constexpr float S = 1.0f / ((float)N * (float)N);
-  const Buf bsrc = MODE == COLS_PRIME ? bwc : bwo;
+  // the new state (zero out of band), scaled, in registers for the three transforms
+  float2 W[PTS];
+#pragma unroll
+  for (int q = 0; q < PTS; ++q) {
+    float2 w;
+    if constexpr (MODE == COLS_PRIME)
+      w = inband(q) ? ld2(bwc, vs, soff_s(q)) : make_float2(0.f, 0.f);
+    else
+      w = lds[swz(t + q * T)];
+    W[q] = cscale(w, S);
+  }
 #pragma unroll 1
   for (int f = 0; f < 3; ++f) {
     // re-derive the lane's indices each field so nothing per-value stays live across transforms
     const int tl = launder(t);
     const int kxl = launder(kx);
-    const uint32_t vsl = ((uint32_t)kxl * Ge::PITCH + (uint32_t)tl) * 8u;
     const uint32_t vml =
         (((uint32_t)(tl >> 1) * Ge::KP + (uint32_t)kxl) * 2u + (uint32_t)(tl & 1)) * 8u;
     const float fkxl = (float)kxl;
@@ -436,8 +448,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
       const int idx = tl + q * T;
-      const bool inb = kxl < KC && (2 * q < PTS ? idx <= K : idx >= N - K);
-      const float2 w = cscale(inb ? ld2(bsrc, vsl, soff_s(q)) : make_float2(0.f, 0.f), S);
+      const float2 w = W[q];
       const float fky = (float)wn(idx, N);
       const float k2 = fkxl * fkxl + fky * fky;
       const float inv = k2 > 0.f ? 1.f / k2 : 0.f;

@@ -39,7 +39,7 @@ def measured_traffic(kernel, n):
     if not os.path.exists(path):
         return None
     prefix = "s2t::%s<%d," % (kernel, n)
-    suffix = ",0,false>" if kernel == "k_cols" else ",false>"
+    suffix = ",0,false>" if kernel == "k_cols" else ",2>"
     with open(path) as f:
         for line in f:
             parts = line.split()

@@ -168,19 +168,29 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
   }
 }
 
-template <int N, int G, bool FORCE>
+// Row kernel variants. ROWS_SINGLE pairs the transforms per row (B' = v'(y0) + i v'(y1) once per
+// row pair, A' = omega' + i u' per row) so that u and v of a row are in registers together, as the
+// CFL reduction and the dye velocity need (stage 0); ROWS_SINGLE_FORCE adds the pointer forcing.
+// ROWS_PAIRED (stages 1 and 2) transforms omega, u and v each for the two rows of a pair at once
+// and packs the products across the pair (u0 omega0 + i u1 omega1, v0 omega0 + i v1 omega1), so
+// every load and store is one 16-byte row-pair element; both variants do five line FFTs per pair.
+enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2 };
+
+template <int N, int G, int KIND>
 __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS;
   constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8);
   __shared__ float2 lds_all[G * L::LDS];
-  // v of the pair's second row, parked by each thread at its own indices while the first row is
-  // processed (holding it in registers instead would cost 16 VGPRs across two transforms)
+  // one real row of the pair (v(y0+1), or omega(y0+1) in ROWS_PAIRED), parked by each thread at
+  // its own indices while the other fields are transformed (holding it in registers instead would
+  // cost 16 VGPRs across two transforms)
   __shared__ float vpark_all[G * N];
   Inst* I = a.inst + blockIdx.y;
   if (!I->st.active) return;  // uniform over the block
   const size_t inst = blockIdx.y;
+  constexpr bool FORCE = KIND == ROWS_SINGLE_FORCE;
   constexpr bool do_force = FORCE;
   const bool do_vel = a.flags & ROW_VEL;
   const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
@@ -193,6 +203,41 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   const bool live = yb < N / 4;
   float2* lds = lds_all + g * L::LDS;
   float* vpark = vpark_all + g * N;
+
+  if constexpr (KIND == ROWS_PAIRED) {
+#pragma unroll 1
+    for (int pair = 0; pair < 2; ++pair) {
+      const int tp = launder(t);
+      const int y0 = yb * 4 + pair * 2;
+      const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
+      const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
+      // omega of rows y0, y0+1: omega0 stays in registers, omega1 is parked
+      float2 X[PTS];
+      row_load<N, true>(X, bu, tp, live, vlo, vhi, 0u, 0u);
+      line_fft<N, +1>(X, lds, tp, a.tw);
+      float om0[PTS];
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        om0[q] = X[q].x;
+        vpark[tp + q * T] = X[q].y;
+      }
+      // f = 0: u pair -> P'(y0), P'(y1); f = 1: v pair -> Q'(y0), Q'(y1)
+#pragma unroll 1
+      for (int f = 0; f < 2; ++f) {
+        const int tr = launder(tp);
+        const uint32_t vlr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tr) * 16u;
+        const uint32_t vhr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tr)) * 16u;
+        row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
+        line_fft<N, +1>(X, lds, tr, a.tw);
+#pragma unroll
+        for (int q = 0; q < PTS; ++q)
+          X[q] = make_float2(X[q].x * om0[q], X[q].y * vpark[tr + q * T]);
+        line_fft<N, -1>(X, lds, tr, a.tw);
+        row_unpack_store<N, true>(X, lds, tr, live, bz, vlr, (uint32_t)f * PLANE_B, 0u);
+      }
+    }
+    return;
+  }
 
   float smax = 0.f;
   bool bad = false;
@@ -582,10 +627,13 @@ struct Impl {
     const int grid = (N / 4 + G - 1) / G;
     prof_begin(s, S2T_PROF_ROWS);
     if (force)
-      hipLaunchKernelGGL((k_rows<N, G, true>), dim3(grid, s->batch), dim3(G * L::T), 0, s->stream,
-                         a);
+      hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE_FORCE>), dim3(grid, s->batch), dim3(G * L::T), 0,
+                         s->stream, a);
+    else if (speed || vel)
+      hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE>), dim3(grid, s->batch), dim3(G * L::T), 0,
+                         s->stream, a);
     else
-      hipLaunchKernelGGL((k_rows<N, G, false>), dim3(grid, s->batch), dim3(G * L::T), 0,
+      hipLaunchKernelGGL((k_rows<N, G, ROWS_PAIRED>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
     check_launch("k_rows");
     prof_end(s);

@@ -3,13 +3,13 @@
 // One right-hand-side evaluation plus its RK stage is two launches (the reference spends ten
 // dispatches on it, src/gpu/solver.js:496-517):
 //
-//   k_rows  (row kernel)   for a block of 4 rows y: inverse row FFTs of the packed half spectra
-//                          A' = omega' + i u' and B' = v'(y0) + i v'(y1) -> physical omega, u, v;
-//                          Z = u omega + i v omega (nl_pack, solver.wgsl:42-54); forward row FFT;
-//                          Hermitian unpack along x into P' = FFT_x(u omega), Q' = FFT_x(v omega)
-//                          for kx = 0..K. Stage 0 also reduces max |u|+|v| for the CFL step
-//                          (reduce.wgsl) and, when asked, writes the dye velocity and transforms
-//                          the pointer forcing (force.wgsl) alongside.
+//   k_rows  (row kernel)   for a pair of rows y0, y0+1: inverse row FFTs of the packed half
+//                          spectra -> physical omega, u, v; products u omega, v omega
+//                          (nl_pack, solver.wgsl:42-54); forward row FFTs; Hermitian unpack along
+//                          x into P' = FFT_x(u omega), Q' = FFT_x(v omega) for kx = 0..K. Stage 0
+//                          also reduces max |u|+|v| for the CFL step (reduce.wgsl) and, when
+//                          asked, writes the dye velocity and transforms the pointer forcing
+//                          (force.wgsl) alongside.
 //   k_cols  (column kernel) for C adjacent columns kx: forward column FFTs of P', Q' ->
 //                          N_hat = i kx P_hat + i ky Q_hat, rhs = (f_hat - N_hat) masked +
 //                          stochastic forcing (assemble, solver.wgsl:56-95); the IF-RK3 update
@@ -17,8 +17,10 @@
 //                          stage's velocity pack (velocity_pack, solver.wgsl:21-40) and inverse
 //                          column FFTs of omega_hat, u_hat, v_hat -> omega', u', v'.
 //
-// Both kernels stage each line in LDS only for the Stockham exchanges of line_fft; all global
-// accesses are coalesced runs (rows contiguous; columns through the 4-row tiles).
+// Both kernels stage each line in LDS only for the Stockham exchanges of line_fft. The mixed
+// fields between them are 2-row tiles (Geo::tile): the row kernel reads and writes a row pair as
+// one contiguous run of 16-byte elements; a column group reads and writes 32-byte runs per row
+// pair, and neighbouring groups on one XCD share the cache lines through L2.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -68,6 +70,10 @@ struct ColArgs {
 template <int N>
 struct RowCfg {
   static constexpr int G = Line<N>::T >= 64 ? 1 : 64 / Line<N>::T;  // row blocks per workgroup
+  // row pairs per row block: one, so a 4096^2 stage is 2048 workgroups over the 768 resident
+  // slots (3 per CU), 2.67 rounds; two pairs per block gave 1.33 rounds and left two thirds of
+  // the CUs idle in the second (measured: 125 -> 115 us per launch)
+  static constexpr int PAIRS = 1;
 };
 template <int N>
 struct ColCfg {
@@ -199,16 +205,17 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   const Buf bv = make_buf(do_vel ? a.vel + inst * a.s_vel : a.vel,
                           do_vel ? (uint32_t)((size_t)N * N * 8) : 0u);
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  constexpr int PAIRS = RowCfg<N>::PAIRS;
   const int yb = blockIdx.x * G + g;
-  const bool live = yb < N / 4;
+  const bool live = yb < N / (2 * PAIRS);
   float2* lds = lds_all + g * L::LDS;
   float* vpark = vpark_all + g * N;
 
   if constexpr (KIND == ROWS_PAIRED) {
 #pragma unroll 1
-    for (int pair = 0; pair < 2; ++pair) {
+    for (int pair = 0; pair < PAIRS; ++pair) {
       const int tp = launder(t);
-      const int y0 = yb * 4 + pair * 2;
+      const int y0 = (yb * PAIRS + pair) * 2;
       const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
       const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
       // omega of rows y0, y0+1: omega0 stays in registers, omega1 is parked
@@ -242,9 +249,9 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   float smax = 0.f;
   bool bad = false;
 #pragma unroll 1
-  for (int pair = 0; pair < 2; ++pair) {
+  for (int pair = 0; pair < PAIRS; ++pair) {
     const int tp = launder(t);
-    const int y0 = yb * 4 + pair * 2;
+    const int y0 = (yb * PAIRS + pair) * 2;
     // lane byte offsets of (row pair y0 / 2, kx = t) and (row pair, kx = T - t) in a plane
     const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
     const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
@@ -624,7 +631,7 @@ struct Impl {
     a.splats = s->splats;
     a.stage = stage;
     a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0);
-    const int grid = (N / 4 + G - 1) / G;
+    const int grid = (N / (2 * RowCfg<N>::PAIRS) + G - 1) / G;
     prof_begin(s, S2T_PROF_ROWS);
     if (force)
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE_FORCE>), dim3(grid, s->batch), dim3(G * L::T), 0,

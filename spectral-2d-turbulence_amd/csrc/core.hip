@@ -348,12 +348,12 @@ void step_once(Solver* s, bool dye) {
   if (force) need_force(s);
   if (dye) need_vel(s);
   ensure_primed(s);
-  s->ops->rows(s, 0, true, dye, force);
-  s->ops->cols(s, 0, force);
+  const bool sd = s->ops->rows(s, 0, true, dye, force);
+  s->ops->cols(s, 0, force, sd);
   s->ops->rows(s, 1, false, false, false);
-  s->ops->cols(s, 1, force);
+  s->ops->cols(s, 1, force, false);
   s->ops->rows(s, 2, false, false, false);
-  s->ops->cols(s, 2, force);
+  s->ops->cols(s, 2, force, false);
   s->phase = (s->phase + 1) % 3;
   if (dye) {
     prof_begin(s, S2T_PROF_DYE);

@@ -33,15 +33,22 @@ struct Geo {
   static __device__ __forceinline__ int jrow(int i) {
     return i <= K ? i : (i >= N - K ? i - (N - NJ) : -1);
   }
-  // Mixed field f of an F-field buffer as planes of 2-row tiles [f][y/2][kx][2]: the values of
-  // rows 2m and 2m+1 at one kx are adjacent (16 bytes). The row kernel, which works on row pairs,
-  // reads a pair of rows as one contiguous run of 16-byte elements; a column group of the column
-  // kernel touches half as many cache lines per access as with row-major planes. 32-bit element
-  // offsets (every buffer here holds fewer than 2^31 elements).
+  // Mixed field f of an F-field buffer as planes of RT-row tiles [f][y/RT][kx][RT]: the values of
+  // rows RT m .. RT m + RT - 1 at one kx are adjacent. The row kernel works on row pairs: a pair is
+  // one 16-byte element per kx, at a stride of RT * 8 bytes; the pairs of one tile are processed by
+  // workgroups that run back to back on one XCD, so they share the tile's cache lines in L2. A
+  // column group of the column kernel touches one cache line per RT rows instead of one per row.
+  // 32-bit element offsets (every buffer here holds fewer than 2^31 elements).
+  static constexpr int RT = 4;
+  static constexpr unsigned KXB = RT * 8;  // bytes from kx to kx + 1 within a tile row
   template <int F>
   static __device__ __forceinline__ unsigned tile(int y, int kx, int f) {
-    return (unsigned)f * (unsigned)PLANE + ((unsigned)(y >> 1) * KP + (unsigned)kx) * 2u +
-           (unsigned)(y & 1);
+    return (unsigned)f * (unsigned)PLANE + ((unsigned)(y / RT) * KP + (unsigned)kx) * RT +
+           (unsigned)(y % RT);
+  }
+  // byte offset of (row y, kx) within a plane
+  static __device__ __forceinline__ unsigned off(int y, int kx) {
+    return (((unsigned)(y / RT) * KP + (unsigned)kx) * RT + (unsigned)(y % RT)) * 8u;
   }
 };
 
@@ -226,6 +233,16 @@ __host__ __device__ __forceinline__ void sincos_f64(double x, double* s, double*
 struct IFactor {
   double mag, c, s;  // E(dt/2) = mag * (c + i s)
 };
+// 1/x from the hardware reciprocal estimate and two Newton steps (within 1 double ulp; used in
+// place of a correctly rounded division, whose scaling and fix-up steps lengthen the per-mode chain)
+__device__ __forceinline__ double rcp_f64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 __device__ __forceinline__ IFactor ifactor_half(double kx, double k2, const Phys& ph, double dt) {
   // (k^2)^p for the integer order p >= 1 (the configuration checks reject other orders)
   double k2p = k2;
@@ -234,7 +251,7 @@ __device__ __forceinline__ IFactor ifactor_half(double kx, double k2, const Phys
   IFactor e;
   e.mag = exp_f64(re * 0.5 * dt);
   if (ph.beta != 0.0) {
-    sincos_f64(ph.beta * kx / k2 * 0.5 * dt, &e.s, &e.c);
+    sincos_f64(ph.beta * kx * rcp_f64(k2) * 0.5 * dt, &e.s, &e.c);
   } else {
     e.s = 0.0;
     e.c = 1.0;
@@ -256,7 +273,8 @@ __device__ __forceinline__ float2 rk_update(int stage, const IFactor& e, float2 
     return make_float2((float)(er * ax - ei * ay), (float)(er * ay + ei * ax));
   }
   if (stage == 1) {
-    const double ir = e.c / e.mag, ii = -e.s / e.mag;  // E(-dt/2)
+    const double im = rcp_f64(e.mag);
+    const double ir = e.c * im, ii = -e.s * im;  // E(-dt/2)
     const double x = 0.75 * (hr * w0.x - hi * w0.y) + 0.25 * (ir * ax - ii * ay);
     const double y = 0.75 * (hr * w0.y + hi * w0.x) + 0.25 * (ir * ay + ii * ax);
     return make_float2((float)x, (float)y);

@@ -130,11 +130,12 @@ struct Server {
 };
 
 static size_t class_bytes(int n, int slots) {
-  // three states + mixed fields per slot, plus per-class scratch
+  // device bytes of a class of `slots` instances of size n (solver_new's allocations)
   const int K = n / 3, KC = K + 1, NJ = 2 * K + 1;
   const size_t pitch = (NJ + 63) / 64 * 64;
   const size_t kp = (KC + 15) / 16 * 16;
   const size_t state = (size_t)KC * pitch * 8, plane = (size_t)n * kp * 8;
+  // per slot: 3 states, 5 mixed planes; per class: 3 full fields and one state of hook scratch
   return (size_t)slots * (3 * state + 5 * plane) + (size_t)n * n * 3 * 8 + state;
 }
 

@@ -19,8 +19,10 @@ struct Ops {
   int twiddle_elems;    // size of the line-FFT twiddle table
   void (*twiddles)(float2* host);  // fill the FP64-derived twiddle table
   // fused step kernels
-  void (*rows)(Solver* s, int stage, bool speed, bool vel, bool force);
-  void (*cols)(Solver* s, int stage, bool force);
+  // rows returns true when the products were written as S', D' (sum / difference form), which
+  // the following cols launch must be told
+  bool (*rows)(Solver* s, int stage, bool speed, bool vel, bool force);
+  void (*cols)(Solver* s, int stage, bool force, bool sd);
   void (*prime)(Solver* s, int i0, int ni);
   void (*rhs)(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
   // full-field helpers

@@ -63,6 +63,7 @@ struct ColArgs {
   size_t s_mz, s_mf, s_mu, s_state;  // per-instance strides (elements)
   int stage;
   int force;
+  int sd;  // the products arrive as S', D' (sum / difference form, ROWS_PAIRED_SD)
   uint32_t step_override;
   double dt_override;
 };
@@ -74,6 +75,7 @@ struct RowCfg {
   // slots (3 per CU), 2.67 rounds; two pairs per block gave 1.33 rounds and left two thirds of
   // the CUs idle in the second (measured: 125 -> 115 us per launch)
   static constexpr int PAIRS = 1;
+  static constexpr int NB = (N / (2 * PAIRS) + G - 1) / G;  // workgroups per instance
 };
 template <int N>
 struct ColCfg {
@@ -116,7 +118,7 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
       const float2 zm = lds[swz((N - k) & (N - 1))];
       float2 p, r;
       herm_unpack(zk, zm, p, r);
-      const uint32_t so = (uint32_t)(q * L::T) * 16u;
+      const uint32_t so = (uint32_t)(q * L::T) * Ge::KXB;
       if constexpr (PAIR) {
         st4(out, p, r, voff, soff0 + so);
       } else {
@@ -132,7 +134,9 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
 // (row pair, kx = T - t) in a 2-row-tiled plane. VPAIR: x = a(y0) + i a(y0+1) for one real field
 // a (plane offset soffa; one 16-byte load per value); otherwise x = a(y) + i b(y) for two real
 // fields of one row (uniform offsets soffa / soffb include the row's 8-byte slot).
-template <int N, bool VPAIR>
+//
+// COMB = +1 / -1 (with VPAIR): the real field is a +- b of two planes (offsets soffa, soffb).
+template <int N, bool VPAIR, int COMB = 0>
 __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int t, bool live,
                                          uint32_t vlo, uint32_t vhi, uint32_t soffa,
                                          uint32_t soffb) {
@@ -146,10 +150,16 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
     if (live) {
       if (2 * q < PTS) {
         if (q * T <= K && k <= K) {
-          const uint32_t so = (uint32_t)(q * T) * 16u;
+          const uint32_t so = (uint32_t)(q * T) * Ge::KXB;
           float2 va, vb;
           if constexpr (VPAIR) {
             ld4(mu, vlo, soffa + so, va, vb);
+            if constexpr (COMB != 0) {
+              float2 wa, wb;
+              ld4(mu, vlo, soffb + so, wa, wb);
+              va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
+              vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
+            }
           } else {
             va = ld2(mu, vlo, soffa + so);
             vb = ld2(mu, vlo, soffb + so);
@@ -159,10 +169,16 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
         }
       } else if ((q + 1) * T - 1 >= N - K && k >= N - K) {
         // m = N - k = (T - t) + (PTS - 1 - q) T
-        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * 16u;
+        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * Ge::KXB;
         float2 va, vb;
         if constexpr (VPAIR) {
           ld4(mu, vhi, soffa + so, va, vb);
+          if constexpr (COMB != 0) {
+            float2 wa, wb;
+            ld4(mu, vhi, soffb + so, wa, wb);
+            va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
+            vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
+          }
         } else {
           va = ld2(mu, vhi, soffa + so);
           vb = ld2(mu, vhi, soffb + so);
@@ -180,7 +196,13 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
 // ROWS_PAIRED (stages 1 and 2) transforms omega, u and v each for the two rows of a pair at once
 // and packs the products across the pair (u0 omega0 + i u1 omega1, v0 omega0 + i v1 omega1), so
 // every load and store is one 16-byte row-pair element; both variants do five line FFTs per pair.
-enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2 };
+// ROWS_PAIRED_SD (stage 0 without dye or pointer forcing) transforms s = u + v and d = u - v in
+// place of u and v: |u| + |v| = max(|s|, |d|), so the CFL maximum is max(max|s|, max|d|), each
+// available from one transform, and the products s omega, d omega are stored as S', D' (the column
+// kernel recovers N_hat = i kx P_hat + i ky Q_hat from them by linearity). Same five FFTs and
+// 16-byte accesses as ROWS_PAIRED. (Holding the u pair through the v transform instead needs
+// ~210 VGPRs; staging it in a scratch row costs 134 MB of HBM traffic per launch.)
+enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2, ROWS_PAIRED_SD = 3 };
 
 template <int N, int G, int KIND>
 __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
@@ -206,18 +228,26 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
                           do_vel ? (uint32_t)((size_t)N * N * 8) : 0u);
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
   constexpr int PAIRS = RowCfg<N>::PAIRS;
-  const int yb = blockIdx.x * G + g;
+  // XCD-aware mapping: consecutive workgroup ids go round-robin over the 8 XCDs; each XCD gets a
+  // contiguous range of row blocks, so the row pairs of one tile run back to back on one XCD
+  constexpr int NB = RowCfg<N>::NB, NB8 = (NB + 7) / 8 * 8;
+  const int lb = (int)(blockIdx.x % 8) * (NB8 / 8) + (int)(blockIdx.x / 8);
+  if (lb >= NB) return;  // uniform over the block
+  const int yb = lb * G + g;
   const bool live = yb < N / (2 * PAIRS);
   float2* lds = lds_all + g * L::LDS;
   float* vpark = vpark_all + g * N;
 
-  if constexpr (KIND == ROWS_PAIRED) {
+  float smax = 0.f;
+  bool bad = false;
+  if constexpr (KIND == ROWS_PAIRED || KIND == ROWS_PAIRED_SD) {
+    constexpr bool SD = KIND == ROWS_PAIRED_SD;
 #pragma unroll 1
     for (int pair = 0; pair < PAIRS; ++pair) {
       const int tp = launder(t);
       const int y0 = (yb * PAIRS + pair) * 2;
-      const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
-      const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
+      const uint32_t vlo = Ge::off(y0, tp);
+      const uint32_t vhi = Ge::off(y0, T - tp);
       // omega of rows y0, y0+1: omega0 stays in registers, omega1 is parked
       float2 X[PTS];
       row_load<N, true>(X, bu, tp, live, vlo, vhi, 0u, 0u);
@@ -229,13 +259,29 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         vpark[tp + q * T] = X[q].y;
       }
       // f = 0: u pair -> P'(y0), P'(y1); f = 1: v pair -> Q'(y0), Q'(y1)
+      // (SD: f = 0: s = u + v -> S', f = 1: d = u - v -> D', with the CFL maximum of |s|, |d|)
 #pragma unroll 1
       for (int f = 0; f < 2; ++f) {
         const int tr = launder(tp);
-        const uint32_t vlr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tr) * 16u;
-        const uint32_t vhr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tr)) * 16u;
-        row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
+        const uint32_t vlr = Ge::off(y0, tr);
+        const uint32_t vhr = Ge::off(y0, T - tr);
+        if constexpr (SD) {
+          if (f == 0)
+            row_load<N, true, +1>(X, bu, tr, live, vlr, vhr, PLANE_B, 2 * PLANE_B);
+          else
+            row_load<N, true, -1>(X, bu, tr, live, vlr, vhr, PLANE_B, 2 * PLANE_B);
+        } else {
+          row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
+        }
         line_fft<N, +1>(X, lds, tr, a.tw);
+        if constexpr (SD) {
+#pragma unroll
+          for (int q = 0; q < PTS; ++q) {
+            const float m0 = fabsf(X[q].x), m1 = fabsf(X[q].y);
+            bad |= !(m0 <= 3.4e38f) || !(m1 <= 3.4e38f);
+            smax = fmaxf(smax, fmaxf(m0, m1));
+          }
+        }
 #pragma unroll
         for (int q = 0; q < PTS; ++q)
           X[q] = make_float2(X[q].x * om0[q], X[q].y * vpark[tr + q * T]);
@@ -243,18 +289,14 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         row_unpack_store<N, true>(X, lds, tr, live, bz, vlr, (uint32_t)f * PLANE_B, 0u);
       }
     }
-    return;
-  }
-
-  float smax = 0.f;
-  bool bad = false;
+  } else {
 #pragma unroll 1
   for (int pair = 0; pair < PAIRS; ++pair) {
     const int tp = launder(t);
     const int y0 = (yb * PAIRS + pair) * 2;
     // lane byte offsets of (row pair y0 / 2, kx = t) and (row pair, kx = T - t) in a plane
-    const uint32_t vlo = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tp) * 16u;
-    const uint32_t vhi = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tp)) * 16u;
+    const uint32_t vlo = Ge::off(y0, tp);
+    const uint32_t vhi = Ge::off(y0, T - tp);
     // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
     float2 B[PTS];
     row_load<N, true>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 0u);
@@ -270,8 +312,8 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
     for (int r = 0; r < 2; ++r) {
       const int tr = launder(tp);
       const int y = y0 + r;
-      const uint32_t vlr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)tr) * 16u;
-      const uint32_t vhr = ((uint32_t)(y0 >> 1) * Ge::KP + (uint32_t)(T - tr)) * 16u;
+      const uint32_t vlr = Ge::off(y0, tr);
+      const uint32_t vhr = Ge::off(y0, T - tr);
       // A' = omega' + i u' of row y (its 8-byte slot r in the row pair)
       float2 A[PTS];
       row_load<N, false>(A, bu, tr, live, vlr, vhr, (uint32_t)r * 8u, PLANE_B + (uint32_t)r * 8u);
@@ -308,6 +350,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
       line_fft<N, -1>(B, lds, tp, a.tw);
       row_unpack_store<N, true>(B, lds, tp, live, bf, vlo, 0u, 0u);
     }
+  }
   }
 
   if (a.flags & ROW_SPEED) {
@@ -360,7 +403,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   const float fkx = (float)kx;
   // per-lane byte offsets: row t of column kx in a 2-row-tiled mixed plane / in the compact state
   // (row block q adds q T rows: q T / 2 row pairs, T is even)
-  const uint32_t vm = (((uint32_t)(t >> 1) * Ge::KP + (uint32_t)kx) * 2u + (uint32_t)(t & 1)) * 8u;
+  const uint32_t vm = Ge::off(t, kx);
   const uint32_t vs = ((uint32_t)kx * Ge::PITCH + (uint32_t)t) * 8u;
   // row block q holds rows t + q T: the low half are the compact rows j = i, the high half
   // j = i - (N - NJ) (T divides N / 2, so a block never spans both halves)
@@ -431,32 +474,38 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     const float fdt = (float)dt;
     const bool need_wc = MODE == COLS_STEP && a.stage != 0;
     const float2 zero = make_float2(0.f, 0.f);
+    // operands of blocks q + 1 (n) and q + 2 (m); a block out of band leaves its set stale, unused
     float2 pn = zero, w0n = zero, wcn = zero, fkn = zero;
-    if (inband(0)) {
-      pn = ld2(bwo, vs, soff_s(0));
-      if (MODE == COLS_STEP) w0n = ld2(bw0, vs, soff_s(0));
-      if (need_wc) wcn = ld2(bwc, vs, soff_s(0));
-      if (FORCE) fkn = ld2(bfk, vs, soff_s(0));
-    }
+    float2 pm = zero, w0m = zero, wcm = zero, fkm = zero;
+    auto fetch = [&](int q, float2& p_, float2& w0_, float2& wc_, float2& fk_) {
+      const uint32_t so = soff_s(q);
+      p_ = ld2(bwo, vs, so);
+      if (MODE == COLS_STEP) w0_ = ld2(bw0, vs, so);
+      if (need_wc) wc_ = ld2(bwc, vs, so);
+      if (FORCE) fk_ = ld2(bfk, vs, so);
+    };
+    if (inband(0)) fetch(0, pn, w0n, wcn, fkn);
+    if (inband(1)) fetch(1, pm, w0m, wcm, fkm);
 #pragma unroll 1
     for (int q = 0; q < PTS; ++q) {
       const float2 pv = pn, w0v = w0n, wcv = wcn, fkv = fkn;
+      pn = pm;
+      w0n = w0m;
+      wcn = wcm;
+      fkn = fkm;
+      if (q + 2 < PTS && inband(q + 2)) fetch(q + 2, pm, w0m, wcm, fkm);
       const bool inb = inband(q);
       float2 wv = zero;
-      if (q + 1 < PTS && inband(q + 1)) {
-        const uint32_t so = soff_s(q + 1);
-        pn = ld2(bwo, vs, so);
-        if (MODE == COLS_STEP) w0n = ld2(bw0, vs, so);
-        if (need_wc) wcn = ld2(bwc, vs, so);
-        if (FORCE) fkn = ld2(bfk, vs, so);
-      }
       if (inb) {
         const int idx = t + q * T;
         const float fky = (float)wn(idx, N);
         const float k2 = fkx * fkx + fky * fky;
         float2 rhs = zero;
         if (k2 > 0.f) {
-          const float2 nh = nhat(pv, lds[swz(idx)], fkx, fky);
+          // N_hat = i kx P_hat + i ky Q_hat; from S_hat = P_hat + Q_hat, D_hat = P_hat - Q_hat it
+          // is i ((kx + ky) S_hat + (kx - ky) D_hat) / 2
+          const float2 nh = a.sd ? nhat(pv, lds[swz(idx)], 0.5f * (fkx + fky), 0.5f * (fkx - fky))
+                                 : nhat(pv, lds[swz(idx)], fkx, fky);
           rhs = make_float2(-nh.x, -nh.y);
           if constexpr (FORCE) rhs = cadd(rhs, fkv);
           if (ph.force_amp > 0.f)
@@ -493,8 +542,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     // re-derive the lane's indices each field so nothing per-value stays live across transforms
     const int tl = launder(t);
     const int kxl = launder(kx);
-    const uint32_t vml =
-        (((uint32_t)(tl >> 1) * Ge::KP + (uint32_t)kxl) * 2u + (uint32_t)(tl & 1)) * 8u;
+    const uint32_t vml = Ge::off(tl, kxl);
     const float fkxl = (float)kxl;
     float2 X[PTS];
 #pragma unroll
@@ -616,7 +664,8 @@ struct Impl {
     if (L::NPASS == 3) put(L::TW2, L::R0 * L::R1, L::R2);
   }
 
-  static void rows(Solver* s, int stage, bool speed, bool vel, bool force) {
+  // returns true when the products were written in the sum / difference form (S', D')
+  static bool rows(Solver* s, int stage, bool speed, bool vel, bool force) {
     RowArgs a{};
     a.mu = s->mixed_u;
     a.mz = s->mixed_z;
@@ -631,19 +680,26 @@ struct Impl {
     a.splats = s->splats;
     a.stage = stage;
     a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0);
-    const int grid = (N / (2 * RowCfg<N>::PAIRS) + G - 1) / G;
+    const int grid = (RowCfg<N>::NB + 7) / 8 * 8;
     prof_begin(s, S2T_PROF_ROWS);
-    if (force)
+    bool sd = false;
+    if (force) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE_FORCE>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
-    else if (speed || vel)
+    } else if (vel) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
-    else
+    } else if (speed) {
+      hipLaunchKernelGGL((k_rows<N, G, ROWS_PAIRED_SD>), dim3(grid, s->batch), dim3(G * L::T), 0,
+                         s->stream, a);
+      sd = true;
+    } else {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_PAIRED>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
+    }
     check_launch("k_rows");
     prof_end(s);
+    return sd;
   }
 
   static ColArgs col_args(Solver* s, int stage) {
@@ -667,9 +723,10 @@ struct Impl {
     return a;
   }
 
-  static void cols(Solver* s, int stage, bool force) {
+  static void cols(Solver* s, int stage, bool force, bool sd) {
     ColArgs a = col_args(s, stage);
     a.force = force ? 1 : 0;
+    a.sd = sd ? 1 : 0;
     constexpr int GROUPS = (Ge::KC + C - 1) / C;
     constexpr int G8 = (GROUPS + 7) / 8 * 8;
     const dim3 grid(G8, s->batch);

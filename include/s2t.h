@@ -269,6 +269,10 @@ int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);
 /* Enable (1) or disable (0) event timing around every kernel launched by s2t_step; enabling
  * clears the totals. */
 int s2t_profile(s2t_solver* h, int enable);
+/* As s2t_profile, timing only the kernel kinds in `mask` (bit k = kind k; 0 disables). Each timed
+ * launch adds two event records to the stream (a few microseconds), so a throughput measurement
+ * times only the kinds it reports. */
+int s2t_profile_kinds(s2t_solver* h, unsigned mask);
 /* Launch count and summed device time (ms) of one kernel kind since profiling was enabled.
  * Waits for queued work. */
 int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);

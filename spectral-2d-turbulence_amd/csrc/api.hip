@@ -317,9 +317,14 @@ int s2t_rhs(s2t_solver* h, uint32_t step_index, double dt, float* rhs) {
 }
 
 int s2t_profile(s2t_solver* h, int enable) {
+  return s2t_profile_kinds(h, enable ? (1u << S2T_PROF_KINDS) - 1 : 0u);
+}
+
+int s2t_profile_kinds(s2t_solver* h, unsigned mask) {
   SOLVER(h);
   S2T_API_BEGIN
-  if (enable) {
+  mask &= (1u << S2T_PROF_KINDS) - 1;
+  if (mask) {
     sync(s);
     s->nev = 0;
     for (int k = 0; k < S2T_PROF_KINDS; ++k) {
@@ -329,7 +334,7 @@ int s2t_profile(s2t_solver* h, int enable) {
   } else {
     prof_collect(s);
   }
-  s->prof = enable != 0;
+  s->prof_mask = mask;
   S2T_API_END
 }
 

@@ -31,7 +31,8 @@ static T* dalloc(size_t elems, const char* what) {
 // profiling: HIP events recorded on the solver's stream around each launch
 // ---------------------------------------------------------------------------------------------
 void prof_begin(Solver* s, int kind) {
-  if (!s->prof) return;
+  s->prof_open = (s->prof_mask >> kind) & 1u;
+  if (!s->prof_open) return;
   if (s->nev == s->cap_ev) {
     const int cap = s->cap_ev ? s->cap_ev * 2 : 256;
     Solver::Ev* ev = (Solver::Ev*)realloc(s->evs, sizeof(Solver::Ev) * cap);
@@ -48,7 +49,8 @@ void prof_begin(Solver* s, int kind) {
 }
 
 void prof_end(Solver* s) {
-  if (!s->prof) return;
+  if (!s->prof_open) return;
+  s->prof_open = false;
   hipchk(hipEventRecord(s->evs[s->nev].b, s->stream), "hipEventRecord");
   s->nev += 1;
 }

@@ -148,16 +148,29 @@ template <> struct Plan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4; }
 template <> struct Plan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8; };
 template <> struct Plan<4096> { static constexpr int R0 = 16, R1 = 16, R2 = 16; };
 
+// Twiddle bases of a radix-R pass. For R = 8 and 16 a thread loads W^{k}, W^{2k}, W^{4k}, W^{8k}
+// (and W^{16k}) and forms the other W^{rk} with one or two complex products (r = a + b or
+// r = a - b of two bases; a few FP32 ulp): 5 loads per radix-16 pass instead of 15. The twiddle
+// loads are vector memory instructions, and the column kernel is bound by its memory-instruction
+// throughput.
+template <int R>
+struct TwBase {
+  static constexpr int NB = R == 16 ? 5 : (R == 8 ? 4 : R - 1);
+  static constexpr int exponent(int j) { return R >= 8 ? 1 << j : j + 1; }
+};
+
 template <int N>
 struct Line {
   static constexpr int R0 = Plan<N>::R0, R1 = Plan<N>::R1, R2 = Plan<N>::R2;
   static constexpr int PTS = R0;       // values per thread
   static constexpr int T = N / R0;     // threads per line
   static constexpr int NPASS = R2 > 1 ? 3 : 2;
-  static constexpr int TW1 = 0;                              // pass-1 table offset
-  static constexpr int TW2 = (R1 - 1) * R0;                  // pass-2 table offset
-  static constexpr int TWSIZE = TW2 + (R2 > 1 ? (R2 - 1) * R0 * R1 : 0);
   static constexpr int LDS = N + N / 32;                     // padded LDS float2 per line
+  // Twiddle tables: pass p (radix R, span NS) stores W_{NS R}^{e k} for k < NS and the exponents
+  // e of TwBase<R> (R >= 8: the powers of two up to R; else every r = 1..R-1).
+  static constexpr int TW1 = 0;                                   // pass-1 table offset
+  static constexpr int TW2 = TwBase<R1>::NB * R0;                 // pass-2 table offset
+  static constexpr int TWSIZE = TW2 + (R2 > 1 ? TwBase<R2>::NB * R0 * R1 : 0);
 };
 
 // LDS slot of line element i: one padding slot per 32 elements. The first exchange writes with
@@ -181,12 +194,34 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
     const int k = b % NS;
 #pragma unroll
     for (int r = 0; r < R; ++r) u[i][r] = lds[swz(b + r * STRIDE)];
+    float2 w[R];  // w[r] = W_{NS R}^{r k}, r = 1..R-1
+    if constexpr (R >= 8) {
+      float2 bs[TwBase<R>::NB];
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      float2 w = tw[(r - 1) * NS + k];
-      if (DIR > 0) w.y = -w.y;
-      u[i][r] = cmul(u[i][r], w);
+      for (int j = 0; j < TwBase<R>::NB; ++j) bs[j] = tw[j * NS + k];
+      w[1] = bs[0];
+      w[2] = bs[1];
+      w[4] = bs[2];
+      w[3] = cmul(bs[1], bs[0]);
+      w[5] = cmul(bs[2], bs[0]);
+      w[6] = cmul(bs[2], bs[1]);
+      w[7] = cmul(bs[3], cconj(bs[0]));
+      if constexpr (R == 16) {
+        w[8] = bs[3];
+        w[9] = cmul(bs[3], bs[0]);
+        w[10] = cmul(bs[3], bs[1]);
+        w[11] = cmul(bs[3], w[3]);
+        w[12] = cmul(bs[3], bs[2]);
+        w[13] = cmul(bs[4], cconj(w[3]));
+        w[14] = cmul(bs[4], cconj(bs[1]));
+        w[15] = cmul(bs[4], cconj(bs[0]));
+      }
+    } else {
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[r] = tw[(r - 1) * NS + k];
     }
+#pragma unroll
+    for (int r = 1; r < R; ++r) u[i][r] = cmul(u[i][r], DIR > 0 ? cconj(w[r]) : w[r]);
     dft<R, DIR>(u[i]);
   }
   if constexpr (LAST) {

@@ -68,7 +68,8 @@ struct Solver {
   bool with_dye = false;
 
   // profiling
-  bool prof = false;
+  unsigned prof_mask = 0;  // kernel kinds timed with events (bit k = S2T_PROF kind k)
+  bool prof_open = false;  // prof_begin recorded an event that prof_end closes
   struct Ev { hipEvent_t a, b; int kind; };
   Ev* evs = nullptr;
   int nev = 0, cap_ev = 0;

@@ -651,17 +651,20 @@ struct Impl {
   static constexpr int G = RowCfg<N>::G;
   static constexpr int C = ColCfg<N>::C;
 
+  // the twiddle tables of Line / TwBase, FP64 angles rounded once
+  template <int R>
+  static void put_pass(float2* h, int off, int ns) {
+    for (int j = 0; j < TwBase<R>::NB; ++j)
+      for (int k = 0; k < ns; ++k) {
+        const long period = (long)ns * R;
+        const long m = ((long)TwBase<R>::exponent(j) * k) % period;
+        const double ang = -2.0 * M_PI * (double)m / (double)period;
+        h[off + j * ns + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
+  }
   static void twiddles(float2* h) {
-    auto put = [&](int off, int ns, int r) {
-      for (int rr = 1; rr < r; ++rr)
-        for (int k = 0; k < ns; ++k) {
-          const long m = (long)rr * k;
-          const double ang = -2.0 * M_PI * (double)m / (double)(ns * r);
-          h[off + (rr - 1) * ns + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-        }
-    };
-    put(L::TW1, L::R0, L::R1);
-    if (L::NPASS == 3) put(L::TW2, L::R0 * L::R1, L::R2);
+    put_pass<L::R1>(h, L::TW1, L::R0);
+    if constexpr (L::NPASS == 3) put_pass<L::R2>(h, L::TW2, L::R0 * L::R1);
   }
 
   // returns true when the products were written in the sum / difference form (S', D')

@@ -180,10 +180,42 @@ struct Line {
 // register and differ by immediate offsets.
 __device__ __forceinline__ int swz(int i) { return i + (i >> 5); }
 
+// Twiddle bases of the butterflies a thread computes in the two passes of a line transform
+// (radix R1 span R0, radix R2 span R0 R1); they depend on the thread only, so a kernel can load
+// them once (load_bases) and pass them to every line_fft.
+template <int N>
+struct Bases {
+  using L = Line<N>;
+  static constexpr int R2E = L::NPASS == 3 ? L::R2 : 2;
+  static constexpr int NB1 = N / (L::R1 * L::T), NB2 = N / (R2E * L::T);
+  static constexpr int NT1 = TwBase<L::R1>::NB, NT2 = L::NPASS == 3 ? TwBase<R2E>::NB : 0;
+  static constexpr int SIZE = NB1 * NT1 + NB2 * NT2;
+};
+
+template <int N>
+__device__ __forceinline__ void load_bases(float2 (&bs)[Bases<N>::SIZE], const float2* __restrict__ tw,
+                                           int t) {
+  using L = Line<N>;
+  using B = Bases<N>;
+#pragma unroll
+  for (int i = 0; i < B::NB1; ++i)
+#pragma unroll
+    for (int j = 0; j < B::NT1; ++j) bs[i * B::NT1 + j] = tw[L::TW1 + j * L::R0 + (t + i * L::T) % L::R0];
+#pragma unroll
+  for (int i = 0; i < B::NB2; ++i)
+#pragma unroll
+    for (int j = 0; j < B::NT2; ++j)
+      bs[B::NB1 * B::NT1 + i * B::NT2 + j] =
+          tw[L::TW2 + j * L::R0 * L::R1 + (t + i * L::T) % (L::R0 * L::R1)];
+}
+
 // Middle/last Stockham pass: NB butterflies of radix R per thread, span NS, inputs from LDS.
-template <int N, int R, int NS, int DIR, bool LAST>
+// Twiddle bases from the table `tw`, or, for BOFF >= 0, from `cached` (this pass's bases start at
+// BOFF).
+template <int N, int R, int NS, int DIR, bool LAST, int BOFF = -1>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2* lds, int t,
-                                              const float2* __restrict__ tw) {
+                                              const float2* __restrict__ tw,
+                                              const float2* cached) {
   constexpr int T = Line<N>::T;
   constexpr int NB = N / (R * T);
   constexpr int STRIDE = N / R;
@@ -198,7 +230,8 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
     if constexpr (R >= 8) {
       float2 bs[TwBase<R>::NB];
 #pragma unroll
-      for (int j = 0; j < TwBase<R>::NB; ++j) bs[j] = tw[j * NS + k];
+      for (int j = 0; j < TwBase<R>::NB; ++j)
+        bs[j] = BOFF >= 0 ? cached[BOFF + i * TwBase<R>::NB + j] : tw[j * NS + k];
       w[1] = bs[0];
       w[2] = bs[1];
       w[4] = bs[2];
@@ -218,7 +251,8 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
       }
     } else {
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[r] = tw[(r - 1) * NS + k];
+      for (int r = 1; r < R; ++r)
+        w[r] = BOFF >= 0 ? cached[BOFF + i * TwBase<R>::NB + r - 1] : tw[(r - 1) * NS + k];
     }
 #pragma unroll
     for (int r = 1; r < R; ++r) u[i][r] = cmul(u[i][r], DIR > 0 ? cconj(w[r]) : w[r]);
@@ -260,10 +294,38 @@ __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds,
   for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
   __syncthreads();
   if constexpr (L::NPASS == 2) {
-    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, twl + L::TW1);
+    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, twl + L::TW1, nullptr);
   } else {
-    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, twl + L::TW1);
-    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, twl + L::TW2);
+    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, twl + L::TW1, nullptr);
+    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, twl + L::TW2, nullptr);
+  }
+}
+
+// The same transform with the twiddle bases already in registers (load_bases).
+template <int N, int DIR>
+__device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds, int t,
+                                         const float2 (&bases)[Bases<N>::SIZE]) {
+  using L = Line<N>;
+  using B = Bases<N>;
+  // opaque copy of the bases: the twiddles derived from them stay local to this transform
+  // instead of being hoisted (all 30 of them) to the caller's scope
+  float2 bs[B::SIZE];
+#pragma unroll
+  for (int i = 0; i < B::SIZE; ++i) {
+    float x = bases[i].x, y = bases[i].y;
+    asm volatile("" : "+v"(x), "+v"(y));
+    bs[i] = make_float2(x, y);
+  }
+  dft<L::R0, DIR>(v);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
+  __syncthreads();
+  if constexpr (L::NPASS == 2) {
+    stockham_pass<N, L::R1, L::R0, DIR, true, 0>(v, lds, t, nullptr, bs);
+  } else {
+    stockham_pass<N, L::R1, L::R0, DIR, false, 0>(v, lds, t, nullptr, bs);
+    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true, B::NB1 * B::NT1>(v, lds, t, nullptr, bs);
   }
 }
 

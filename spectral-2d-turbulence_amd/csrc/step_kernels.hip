@@ -237,6 +237,21 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   const bool live = yb < N / (2 * PAIRS);
   float2* lds = lds_all + g * L::LDS;
   float* vpark = vpark_all + g * N;
+  // Twiddle bases of this thread, loaded once for all transforms of the kernel: only where the
+  // LDS limit (three workgroups per CU at N = 4096) leaves the registers free, in the paired
+  // variant (146 VGPRs with them; the stage-0 variants would exceed 168, smaller N would lose
+  // occupancy).
+  constexpr bool TWREG = KIND == ROWS_PAIRED && N == 4096;
+  float2 tb[Bases<N>::SIZE];
+  if constexpr (TWREG) load_bases<N>(tb, a.tw, t);
+  auto fft_inv = [&](float2 (&x)[PTS], int tt) {
+    if constexpr (TWREG) line_fft<N, +1>(x, lds, tt, tb);
+    else line_fft<N, +1>(x, lds, tt, a.tw);
+  };
+  auto fft_fwd = [&](float2 (&x)[PTS], int tt) {
+    if constexpr (TWREG) line_fft<N, -1>(x, lds, tt, tb);
+    else line_fft<N, -1>(x, lds, tt, a.tw);
+  };
 
   float smax = 0.f;
   bool bad = false;
@@ -251,7 +266,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
       // omega of rows y0, y0+1: omega0 stays in registers, omega1 is parked
       float2 X[PTS];
       row_load<N, true>(X, bu, tp, live, vlo, vhi, 0u, 0u);
-      line_fft<N, +1>(X, lds, tp, a.tw);
+      fft_inv(X, tp);
       float om0[PTS];
 #pragma unroll
       for (int q = 0; q < PTS; ++q) {
@@ -273,7 +288,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         } else {
           row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
         }
-        line_fft<N, +1>(X, lds, tr, a.tw);
+        fft_inv(X, tr);
         if constexpr (SD) {
 #pragma unroll
           for (int q = 0; q < PTS; ++q) {
@@ -285,7 +300,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
 #pragma unroll
         for (int q = 0; q < PTS; ++q)
           X[q] = make_float2(X[q].x * om0[q], X[q].y * vpark[tr + q * T]);
-        line_fft<N, -1>(X, lds, tr, a.tw);
+        fft_fwd(X, tr);
         row_unpack_store<N, true>(X, lds, tr, live, bz, vlr, (uint32_t)f * PLANE_B, 0u);
       }
     }
@@ -300,7 +315,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
     // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
     float2 B[PTS];
     row_load<N, true>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 0u);
-    line_fft<N, +1>(B, lds, tp, a.tw);
+    fft_inv(B, tp);
     float v0[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
@@ -317,7 +332,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
       // A' = omega' + i u' of row y (its 8-byte slot r in the row pair)
       float2 A[PTS];
       row_load<N, false>(A, bu, tr, live, vlr, vhr, (uint32_t)r * 8u, PLANE_B + (uint32_t)r * 8u);
-      line_fft<N, +1>(A, lds, tr, a.tw);
+      fft_inv(A, tr);
       // physical omega = Re A, u = Im A, v = Re/Im B; packed product Z = u omega + i v omega
       const uint32_t vvel = ((uint32_t)y * N + (uint32_t)tr) * 8u;
 #pragma unroll
@@ -332,7 +347,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         if (do_vel && live) st2(bv, make_float2(u, v), vvel, (uint32_t)(q * T) * 8u);
         A[q] = make_float2(u * om, v * om);
       }
-      line_fft<N, -1>(A, lds, tr, a.tw);
+      fft_fwd(A, tr);
       row_unpack_store<N, false>(A, lds, tr, live, bz, vlr, (uint32_t)r * 8u,
                                  PLANE_B + (uint32_t)r * 8u);
     }
@@ -347,7 +362,7 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         B[q] = live ? make_float2(pointer_curl(xu, yu0, a.splats), pointer_curl(xu, yu1, a.splats))
                     : make_float2(0.f, 0.f);
       }
-      line_fft<N, -1>(B, lds, tp, a.tw);
+      fft_fwd(B, tp);
       row_unpack_store<N, true>(B, lds, tp, live, bf, vlo, 0u, 0u);
     }
   }

@@ -1,0 +1,70 @@
+"""Configuration checks, kernel-kind profiling, and the two stage-0 row-kernel forms.
+
+Stage 0 of a step transforms (omega, s = u + v, d = u - v) row pairs when no dye velocity is
+needed, and (v pair, omega + i u per row) when it is; both give the same step to FP32 round-off
+and the same CFL maximum |u| + |v| = max(|s|, |d|).
+"""
+
+import numpy as np
+import pytest
+
+from oracle.spectral import random_field
+
+pytestmark = pytest.mark.gpu
+
+PHYS = dict(beta=8.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5, dt_max=0.02)
+
+
+@pytest.mark.parametrize("bad, msg", [
+    (dict(hyper_p=2.5), "hyper_p must be an integer"),
+    (dict(hyper_p=0.0), "hyper_p must be an integer"),
+    (dict(hyper_p=17.0), "hyper_p must be an integer"),
+    (dict(drag=-0.1), "drag must be non-negative"),
+    (dict(viscosity=-1.0), "viscosity must be non-negative"),
+    (dict(beta=1e6, dt_max=0.02), "beta"),
+    (dict(beta=100.0, fixed_dt=200.0), "beta"),
+])
+def test_parameter_ranges(s2t, bad, msg):
+    with s2t.Solver(64) as s:
+        with pytest.raises(s2t.S2TError, match=msg):
+            s.configure(**dict(PHYS, **bad))
+        s.configure(**dict(PHYS, hyper_p=16.0))  # the largest accepted order
+        s.configure(**PHYS)
+
+
+def test_profile_kinds(s2t):
+    with s2t.Solver(128) as s:
+        s.configure(**PHYS)
+        s.seed_spectrum(k0=8.0, amplitude=5.0, seed=3)
+        s.step(1)
+        s.profile(True, kinds=[s2t.PROF_COLS])
+        s.step(2)
+        s.profile(False)
+        cols, rows = s.profile_read(s2t.PROF_COLS), s.profile_read(s2t.PROF_ROWS)
+        assert cols[0] == 6 and cols[1] > 0
+        assert rows[0] == 0 and rows[1] == 0
+        s.profile(True)
+        s.step(1)
+        s.profile(False)
+        assert s.profile_read(s2t.PROF_ROWS)[0] == 3 and s.profile_read(s2t.PROF_COLS)[0] == 3
+
+
+@pytest.mark.parametrize("n", [128, 1024])
+def test_stage0_forms_agree(s2t, n):
+    """One adaptive step with the sum / difference stage 0 (no dye) and with the per-row stage 0
+    (dye on): same dt to FP32 round-off of the speed, same state to 2e-6 relative."""
+    w = random_field(n, k0=n / 16, seed=11, amplitude=6.0)
+    out = {}
+    for dye in (False, True):
+        with s2t.Solver(n) as s:
+            s.configure(**PHYS)
+            s.seed_physical(w)
+            s.step(1, dye=dye)
+            d = s.diagnostics()
+            out[dye] = (s.read_state(), d["dt"], d["maxSpeed"])
+    (a, dta, spa), (b, dtb, spb) = out[False], out[True]
+    assert abs(spa / spb - 1) < 1e-6
+    assert abs(dta / dtb - 1) < 1e-6
+    err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+    print("n=%d stage-0 forms: speed %.9g vs %.9g, state rel diff %.2e" % (n, spa, spb, err))
+    assert err < 2e-6

@@ -313,7 +313,7 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: forced beta-plane turbulence from rest, state resident in HBM",
-        "config": {"workload": "forced beta-plane turbulence %d^2 FP32 (config 3)" % n, "n": n,
+        "config": {"workload": "forced beta-plane turbulence %d^2 FP32" % n, "n": n,
                    "beta": 10.0, "drag": 0.01, "hyper_p": 4, "viscosity_rate": 1.0,
                    "force_k": 32, "force_width": 2, "injection_rate": 1e-3,
                    "force_amp": round(params["force_amp"], 6), "cfl": 0.5,

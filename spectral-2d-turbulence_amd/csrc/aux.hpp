@@ -10,7 +10,6 @@ namespace s2t {
 
 void fill_u32(hipStream_t st, void* p, size_t words, uint32_t v);
 void reset_inst(hipStream_t st, Inst* in, int i0, int ni);
-void step_end(hipStream_t st, Inst* in, int ni);
 void vpack_full(hipStream_t st, const float2* w, float2* pa, float2* pb, int n);
 void assemble_full(hipStream_t st, const float2* z, const float2* fhat, float2* rhs,
                    const Phys& ph, uint32_t step, double dt, int n);

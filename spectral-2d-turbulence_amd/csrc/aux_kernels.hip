@@ -52,35 +52,16 @@ __global__ void k_reset_inst(Inst* in, int i0, int ni) {
   d.dt = in[i0 + i].ph.dt_max;
   d.t = 0.0;
   d.speed = 0.f;
-  d.speed_bits = 0u;
+  for (int k = 0; k < DevState::SPEED_SLOTS; ++k) d.speed_bits[k] = 0u;
   d.step = 0u;
   d.nonfinite = 0u;
   d.active = 1;
   d.steps_left = -1;
+  d.cols_done = 0u;
 }
 void reset_inst(hipStream_t st, Inst* in, int i0, int ni) {
   hipLaunchKernelGGL(k_reset_inst, dim3(blocks(ni, 64)), dim3(64), 0, st, in, i0, ni);
   check_launch("k_reset_inst");
-}
-
-// per-step epilogue: advance stepIndex and time, clear the speed maximum for the next step and
-// retire instances whose step budget is spent (stepIndex++ of solver.js:520)
-__global__ void k_step_end(Inst* in, int ni) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ni) return;
-  DevState& d = in[i].st;
-  if (!d.active) return;
-  d.step += 1u;
-  d.t += d.dt;
-  d.speed_bits = 0u;
-  if (d.steps_left > 0) {
-    d.steps_left -= 1;
-    if (d.steps_left == 0) d.active = 0;
-  }
-}
-void step_end(hipStream_t st, Inst* in, int ni) {
-  hipLaunchKernelGGL(k_step_end, dim3(blocks(ni, 64)), dim3(64), 0, st, in, ni);
-  check_launch("k_step_end");
 }
 
 // plain streaming copy (16-byte vectors, grid-stride): the reference kernel whose counter readings

@@ -364,9 +364,6 @@ void step_once(Solver* s, bool dye) {
     prof_end(s);
     s->dye_parity = 1 - s->dye_parity;
   }
-  prof_begin(s, S2T_PROF_OTHER);
-  step_end(s->stream, s->inst, s->batch);
-  prof_end(s);
 }
 
 }  // namespace s2t

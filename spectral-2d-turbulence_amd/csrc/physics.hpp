@@ -79,18 +79,35 @@ struct Splats {
 
 // Per-instance device scalars. dt and dt-derived values never leave the device during stepping
 // (README.md:132-134 of the reference); the host reads them with the diagnostics. Only the
-// per-step epilogue kernel (k_step_end) advances step, t and steps_left, so no kernel reads a
-// field that another block of the same launch writes.
+// per-step epilogue (step_epilogue, run by the last column workgroup of stage 2 to finish, after
+// every other workgroup of that launch has counted itself done) advances step, t and steps_left,
+// so no workgroup reads a field that another workgroup of the same launch is writing.
 struct DevState {
   double dt;            // step of the step in flight / last step
   double t;             // simulated time
   float speed;          // max |u|+|v| of the last step's initial field
-  unsigned speed_bits;  // running max for the step in flight (float bits, non-negative)
+  // running max of |u|+|v| for the step in flight (float bits, non-negative), spread over slots
+  // by row block so the per-workgroup atomics of a launch do not all hit one address
+  static constexpr int SPEED_SLOTS = 64;
+  unsigned speed_bits[SPEED_SLOTS];
   unsigned step;        // steps completed since the last reset = stepIndex of the step in flight
   unsigned nonfinite;   // sticky flag: a non-finite speed was seen
   int active;           // 0: the step kernels skip this instance
   int steps_left;       // steps until the instance retires itself (< 0: no limit)
+  unsigned cols_done;   // column workgroups of the last stage that have finished (then reset)
 };
+
+// Per-step epilogue of one instance: advance stepIndex and time, clear the speed maximum for the
+// next step and retire the instance when its step budget is spent (stepIndex++ of solver.js:520).
+// Run by the last column workgroup of stage 2 to finish (which also clears speed_bits).
+__device__ __forceinline__ void step_epilogue(DevState& d) {
+  d.step += 1u;
+  d.t += d.dt;
+  if (d.steps_left > 0) {
+    d.steps_left -= 1;
+    if (d.steps_left == 0) d.active = 0;
+  }
+}
 
 // One simulation instance's parameters and scalars, an array of these per solver (a plain solver
 // has one; a batch has one per member). Kernels index it with blockIdx.y.

@@ -4,7 +4,7 @@
 // thread admits each into a free slot of the class of its grid size, seeds it on the device and
 // sets its step budget; every active class is then stepped on its own stream (one launch per
 // fused kernel covers all slots of a class; idle slots exit at once). Each slot retires itself
-// on the device after exactly its budgeted steps (k_step_end), so a result does not depend on
+// on the device after exactly its budgeted steps (the step epilogue), so a result does not depend on
 // when the host polls or on which other requests share the class: it is bit-identical to the same
 // request run alone. Completions record their latency and FP64 diagnostics (and, on request, the
 // final spectrum); the server reports p50/p99 latency and aggregate throughput.

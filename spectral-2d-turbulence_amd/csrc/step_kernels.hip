@@ -369,9 +369,17 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   }
 
   if (a.flags & ROW_SPEED) {
+    // wave max, workgroup max through LDS, one atomic per workgroup into the block's slot
+    __shared__ float wmax[G * L::T / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmaxf(smax, __shfl_xor(smax, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&I->st.speed_bits, __float_as_uint(smax));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = smax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = wmax[0];
+      for (int w = 1; w < G * L::T / 64; ++w) m = fmaxf(m, wmax[w]);
+      atomicMax(&I->st.speed_bits[blockIdx.x % DevState::SPEED_SLOTS], __float_as_uint(m));
+    }
     if (bad) atomicOr(&I->st.nonfinite, 1u);
   }
 }
@@ -439,7 +447,16 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     } else {
       step = I->st.step;
       if (a.stage == 0) {
-        const float speed = __uint_as_float(I->st.speed_bits);
+        // the step's speed maximum: max over the row kernel's slots
+        __shared__ float speed_s;
+        if (tid < 64) {
+          float v = __uint_as_float(I->st.speed_bits[tid % DevState::SPEED_SLOTS]);
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+          if (tid == 0) speed_s = v;
+        }
+        __syncthreads();
+        const float speed = speed_s;
         dt = ph.fixed_dt > 0.0 ? ph.fixed_dt : cfl_dt(speed, ph);
         if (bid == 0 && tid == 0) {
           I->st.dt = dt;
@@ -575,6 +592,30 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     if (kxl < KC) {
 #pragma unroll
       for (int q = 0; q < PTS; ++q) st2(bu, X[q], vml, (uint32_t)f * PLANE_B + q * ROWS_B);
+    }
+  }
+  if constexpr (MODE == COLS_STEP) {
+    if (a.stage == 2) {
+      // The last column workgroup of the instance to finish runs the per-step epilogue. It only
+      // has to come after every workgroup's reads of the instance scalars (step, dt, active),
+      // which each wave consumed at its start, so the counter is a relaxed atomic with no fence:
+      // an agent-scope fence per workgroup writes back the XCD's L2 and cost about 30 us per
+      // launch at 4096^2. The next launch sees the epilogue's stores through the kernel boundary.
+      __syncthreads();
+      if (tid < 64) {  // wave 0
+        unsigned last = 0u;
+        if (tid == 0)
+          last = atomicAdd(&I->st.cols_done, 1u) == (unsigned)(GROUPS - 1) ? 1u : 0u;
+        last = __shfl(last, 0);
+        if (last) {
+          if (tid < DevState::SPEED_SLOTS)
+            __hip_atomic_store(&I->st.speed_bits[tid], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (tid == 0) {
+            step_epilogue(I->st);
+            __hip_atomic_store(&I->st.cols_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      }
     }
   }
 }

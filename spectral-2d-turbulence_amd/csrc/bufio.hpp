@@ -15,6 +15,11 @@ namespace s2t {
 // word 3 of a raw (stride 0, untyped) buffer resource on gfx9
 constexpr int BUF_WORD3 = 0x00020000;
 
+// A per-lane offset beyond every resource's range (and beyond it plus any SGPR offset used here):
+// the load returns zero and the store is dropped, with no memory access. Predicating an access
+// by its offset keeps it out of a branch, so the wait counts of the loads around it stay exact.
+constexpr uint32_t BUF_OOB = 0x80000000u;
+
 struct Buf {
   __amdgpu_buffer_rsrc_t r;
 };

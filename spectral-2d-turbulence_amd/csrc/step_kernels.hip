@@ -406,7 +406,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   if (cg >= GROUPS) return;
   Inst* I = a.inst + blockIdx.y;
   if (!I->st.active) return;  // uniform over the block
-  const Phys& ph = I->ph;
+  // a copy taken before the kernel's first store: the loads are known unclobbered, so they are
+  // scalar loads and the parameters stay in SGPRs (a reference would be re-read from memory after
+  // every buffer store, each time with a wait for all loads in flight)
+  const Phys ph = I->ph;
   const size_t inst = blockIdx.y;
   const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
   const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
@@ -502,37 +505,34 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     }
 
     // per-mode right-hand side and RK stage, one row block at a time (rolled, so only one
-    // block's values are live); the global operands of block q + 1 are fetched during block q
+    // block's values are live). The loop body covers two blocks with two operand sets, A for even
+    // and B for odd q: the operands of block q + 1 are fetched before block q is computed, into
+    // the set block q does not use, so no register copies wait on loads in flight.
+    static_assert(PTS % 2 == 0, "the mode loop takes row blocks in pairs");
     const float fdt = (float)dt;
     const bool need_wc = MODE == COLS_STEP && a.stage != 0;
     const float2 zero = make_float2(0.f, 0.f);
-    // operands of blocks q + 1 (n) and q + 2 (m); a block out of band leaves its set stale, unused
-    float2 pn = zero, w0n = zero, wcn = zero, fkn = zero;
-    float2 pm = zero, w0m = zero, wcm = zero, fkm = zero;
+    // Loads and stores of lanes out of band take an offset out of range (no branch around them).
+    float2 pA = zero, w0A = zero, wcA = zero, fkA = zero;
+    float2 pB = zero, w0B = zero, wcB = zero, fkB = zero;
     auto fetch = [&](int q, float2& p_, float2& w0_, float2& wc_, float2& fk_) {
-      const uint32_t so = soff_s(q);
-      p_ = ld2(bwo, vs, so);
-      if (MODE == COLS_STEP) w0_ = ld2(bw0, vs, so);
-      if (need_wc) wc_ = ld2(bwc, vs, so);
-      if (FORCE) fk_ = ld2(bfk, vs, so);
+      const uint32_t so = soff_s(q), v = q < PTS && inband(q) ? vs : BUF_OOB;
+      p_ = ld2(bwo, v, so);
+      if (MODE == COLS_STEP) {
+        w0_ = ld2(bw0, v, so);
+        // issued at stage 0 too (reading zero): a load under a run-time condition makes the
+        // compiler merge the untaken case into the register at once, waiting for the other loads
+        wc_ = ld2(bwc, need_wc ? v : BUF_OOB, so);
+      }
+      if (FORCE) fk_ = ld2(bfk, v, so);
     };
-    if (inband(0)) fetch(0, pn, w0n, wcn, fkn);
-    if (inband(1)) fetch(1, pm, w0m, wcm, fkm);
-#pragma unroll 1
-    for (int q = 0; q < PTS; ++q) {
-      const float2 pv = pn, w0v = w0n, wcv = wcn, fkv = fkn;
-      pn = pm;
-      w0n = w0m;
-      wcn = wcm;
-      fkn = fkm;
-      if (q + 2 < PTS && inband(q + 2)) fetch(q + 2, pm, w0m, wcm, fkm);
+    auto block = [&](int q, const float2 pv, const float2 w0v, const float2 wcv, const float2 fkv) {
+      float2 wv = zero, rhs = zero;
       const bool inb = inband(q);
-      float2 wv = zero;
       if (inb) {
         const int idx = t + q * T;
         const float fky = (float)wn(idx, N);
         const float k2 = fkx * fkx + fky * fky;
-        float2 rhs = zero;
         if (k2 > 0.f) {
           // N_hat = i kx P_hat + i ky Q_hat; from S_hat = P_hat + Q_hat, D_hat = P_hat - Q_hat it
           // is i ((kx + ky) S_hat + (kx - ky) D_hat) / 2
@@ -547,10 +547,18 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
             wv = rk_update(a.stage, e, w0v, a.stage == 0 ? w0v : wcv, rhs, dt);
           }
         }
-        st2(bwo, MODE == COLS_RHS ? rhs : wv, vs, soff_s(q));
       }
+      st2(bwo, MODE == COLS_RHS ? rhs : wv, inb ? vs : BUF_OOB, soff_s(q));
       // the new state replaces Q_hat in this thread's own slot of the exchange buffer
       if constexpr (MODE == COLS_STEP) lds[swz(t + q * T)] = wv;
+    };
+    fetch(0, pA, w0A, wcA, fkA);
+#pragma unroll 1
+    for (int q = 0; q < PTS; q += 2) {
+      fetch(q + 1, pB, w0B, wcB, fkB);
+      block(q, pA, w0A, wcA, fkA);
+      fetch(q + 2, pA, w0A, wcA, fkA);  // past the last block: reads nothing (no branch)
+      block(q + 1, pB, w0B, wcB, fkB);
     }
     if constexpr (MODE == COLS_RHS) return;
   }
@@ -583,7 +591,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       const float2 w = W[q];
       const float fky = (float)wn(idx, N);
       const float k2 = fkxl * fkxl + fky * fky;
-      const float inv = k2 > 0.f ? 1.f / k2 : 0.f;
+      // hardware reciprocal (1 ulp; the IEEE quotient takes a dozen instructions per value)
+      const float inv = k2 > 0.f ? __builtin_amdgcn_rcpf(k2) : 0.f;
       // u_hat = i ky / k^2 omega_hat, v_hat = -i kx / k^2 omega_hat
       const float g = (f == 1 ? fky : -fkxl) * inv;
       X[q] = f == 0 ? w : cscale(cmuli(w), g);

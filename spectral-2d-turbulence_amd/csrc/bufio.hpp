@@ -49,14 +49,14 @@ __device__ __forceinline__ void ld4(Buf b, uint32_t voff, uint32_t soff, float2&
   hi = make_float2(__uint_as_float(v[2]), __uint_as_float(v[3]));
 }
 
+// The 16-byte element is written as two 8-byte stores. A store of more than 8 bytes reads its
+// data VGPRs after it issues, and for gfx950 the compiler does not keep a VALU write of those
+// VGPRs away from it: with stores issued back to back (no branch between them) a write landed
+// one instruction after the store and corrupted the stored values now and then (results that
+// differed run to run). 8-byte stores are not affected.
 __device__ __forceinline__ void st4(Buf b, float2 lo, float2 hi, uint32_t voff, uint32_t soff) {
-  typedef unsigned u32x4 __attribute__((vector_size(16)));
-  u32x4 v;
-  v[0] = __float_as_uint(lo.x);
-  v[1] = __float_as_uint(lo.y);
-  v[2] = __float_as_uint(hi.x);
-  v[3] = __float_as_uint(hi.y);
-  __builtin_amdgcn_raw_buffer_store_b128(v, b.r, (int)voff, (int)soff, 0);
+  st2(b, lo, voff, soff);
+  st2(b, hi, voff, soff + 8u);
 }
 
 // Opaque copy of a value: loop-invariant code that depends on it cannot be hoisted out of the

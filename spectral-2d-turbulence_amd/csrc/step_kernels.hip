@@ -113,17 +113,18 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
 #pragma unroll
   for (int q = 0; q < L::PTS; ++q) {
     const int k = t + q * L::T;
-    if (q * L::T <= Ge::K && k <= Ge::K) {
+    if (q * L::T <= Ge::K) {
+      // lanes past kx = K store through an out-of-range offset (dropped) rather than branching
       const float2 zk = lds[swz(k)];
       const float2 zm = lds[swz((N - k) & (N - 1))];
       float2 p, r;
       herm_unpack(zk, zm, p, r);
-      const uint32_t so = (uint32_t)(q * L::T) * Ge::KXB;
+      const uint32_t so = (uint32_t)(q * L::T) * Ge::KXB, v = k <= Ge::K ? voff : BUF_OOB;
       if constexpr (PAIR) {
-        st4(out, p, r, voff, soff0 + so);
+        st4(out, p, r, v, soff0 + so);
       } else {
-        st2(out, p, voff, soff0 + so);
-        st2(out, r, voff, soff1 + so);
+        st2(out, p, v, soff0 + so);
+        st2(out, r, v, soff1 + so);
       }
     }
   }
@@ -143,48 +144,39 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K;
+  // Lanes whose k is out of band load through an out-of-range offset (zero, no memory access)
+  // instead of branching around the load: a load under a per-lane branch makes the compiler wait
+  // for it before the branch merges, which serialised the PTS loads of a line.
+  auto pair = [&](uint32_t v, uint32_t so, float2& va, float2& vb) {
+    if constexpr (VPAIR) {
+      ld4(mu, v, soffa + so, va, vb);
+      if constexpr (COMB != 0) {
+        float2 wa, wb;
+        ld4(mu, v, soffb + so, wa, wb);
+        va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
+        vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
+      }
+    } else {
+      va = ld2(mu, v, soffa + so);
+      vb = ld2(mu, v, soffb + so);
+    }
+  };
 #pragma unroll
   for (int q = 0; q < PTS; ++q) {
     const int k = t + q * T;
     float2 v = make_float2(0.f, 0.f);
-    if (live) {
-      if (2 * q < PTS) {
-        if (q * T <= K && k <= K) {
-          const uint32_t so = (uint32_t)(q * T) * Ge::KXB;
-          float2 va, vb;
-          if constexpr (VPAIR) {
-            ld4(mu, vlo, soffa + so, va, vb);
-            if constexpr (COMB != 0) {
-              float2 wa, wb;
-              ld4(mu, vlo, soffb + so, wa, wb);
-              va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
-              vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
-            }
-          } else {
-            va = ld2(mu, vlo, soffa + so);
-            vb = ld2(mu, vlo, soffb + so);
-          }
-          // k = 0: the real parts (the DC column of a real field)
-          v = k == 0 ? make_float2(va.x, vb.x) : make_float2(va.x - vb.y, va.y + vb.x);
-        }
-      } else if ((q + 1) * T - 1 >= N - K && k >= N - K) {
-        // m = N - k = (T - t) + (PTS - 1 - q) T
-        const uint32_t so = (uint32_t)((PTS - 1 - q) * T) * Ge::KXB;
+    if (2 * q < PTS) {
+      if (q * T <= K) {
         float2 va, vb;
-        if constexpr (VPAIR) {
-          ld4(mu, vhi, soffa + so, va, vb);
-          if constexpr (COMB != 0) {
-            float2 wa, wb;
-            ld4(mu, vhi, soffb + so, wa, wb);
-            va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
-            vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
-          }
-        } else {
-          va = ld2(mu, vhi, soffa + so);
-          vb = ld2(mu, vhi, soffb + so);
-        }
-        v = make_float2(va.x + vb.y, vb.x - va.y);
+        pair(live && k <= K ? vlo : BUF_OOB, (uint32_t)(q * T) * Ge::KXB, va, vb);
+        // k = 0: the real parts (the DC column of a real field)
+        v = k == 0 ? make_float2(va.x, vb.x) : make_float2(va.x - vb.y, va.y + vb.x);
       }
+    } else if ((q + 1) * T - 1 >= N - K) {
+      // m = N - k = (T - t) + (PTS - 1 - q) T
+      float2 va, vb;
+      pair(live && k >= N - K ? vhi : BUF_OOB, (uint32_t)((PTS - 1 - q) * T) * Ge::KXB, va, vb);
+      v = make_float2(va.x + vb.y, vb.x - va.y);
     }
     x[q] = v;
   }

@@ -379,6 +379,27 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
 // ---------------------------------------------------------------------------------------------
 // column kernel
 // ---------------------------------------------------------------------------------------------
+// Row blocks of a column line (thread t, block q holds row i = t + q T) that lie out of band for
+// every t: K < i < N - K for all of them, a run in the middle (q = 6..9 of 16 at N = 4096). The
+// column kernel's mode loop takes blocks in pairs and skips the pairs inside this run.
+template <int N>
+struct ColBand {
+  static constexpr int T = Line<N>::T, PTS = Line<N>::PTS, K = Geo<N>::K;
+  static constexpr bool out(int q) { return q * T > K && (q + 1) * T - 1 < N - K; }
+  static constexpr int skip_first() {
+    for (int q = 0; q + 1 < PTS; q += 2)
+      if (out(q) && out(q + 1)) return q;
+    return PTS;
+  }
+  static constexpr int skip_count() {
+    int n = 0;
+    for (int q = skip_first(); q + 1 < PTS && out(q) && out(q + 1); q += 2) n += 2;
+    return n;
+  }
+};
+static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() == 4,
+              "4096: blocks 6..9 are out of band");
+
 template <int N, int C, int MODE, bool FORCE>
 __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   using L = Line<N>;
@@ -432,6 +453,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     const int i = t + q * T;
     return active && (2 * q < PTS ? i <= K : i >= N - K);
   };
+  // row blocks out of band for every lane: neither parked, computed nor transformed
+  auto qout = [](int q) constexpr { return ColBand<N>::out(q); };
 
   if constexpr (MODE != COLS_PRIME) {
     double dt;
@@ -482,7 +505,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       line_fft<N, -1>(P, lds, t, a.tw);
 #pragma unroll
       for (int q = 0; q < PTS; ++q)
-        if (inband(q)) st2(bwo, P[q], vs, soff_s(q));
+        if (!qout(q) && inband(q)) st2(bwo, P[q], vs, soff_s(q));
     }
     {
       // Q_hat parks in the exchange buffer, each thread at its own indices
@@ -493,7 +516,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       line_fft<N, -1>(Q, lds, t, a.tw);
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < PTS; ++q) lds[swz(t + q * T)] = Q[q];
+      for (int q = 0; q < PTS; ++q)
+        if (!qout(q)) lds[swz(t + q * T)] = Q[q];
     }
 
     // per-mode right-hand side and RK stage, one row block at a time (rolled, so only one
@@ -544,12 +568,17 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       // the new state replaces Q_hat in this thread's own slot of the exchange buffer
       if constexpr (MODE == COLS_STEP) lds[swz(t + q * T)] = wv;
     };
-    fetch(0, pA, w0A, wcA, fkA);
+    // pairs of blocks, skipping the pairs out of band (pair pi starts at block qof(pi))
+    constexpr int SK0 = ColBand<N>::skip_first(), SKN = ColBand<N>::skip_count();
+    constexpr int NPI = (PTS - SKN) / 2;
+    auto qof = [](int pi) { return 2 * pi + (2 * pi >= SK0 ? SKN : 0); };
+    fetch(qof(0), pA, w0A, wcA, fkA);
 #pragma unroll 1
-    for (int q = 0; q < PTS; q += 2) {
+    for (int pi = 0; pi < NPI; ++pi) {
+      const int q = qof(pi);
       fetch(q + 1, pB, w0B, wcB, fkB);
       block(q, pA, w0A, wcA, fkA);
-      fetch(q + 2, pA, w0A, wcA, fkA);  // past the last block: reads nothing (no branch)
+      fetch(qof(pi + 1), pA, w0A, wcA, fkA);  // past the last block: reads nothing (no branch)
       block(q + 1, pB, w0B, wcB, fkB);
     }
     if constexpr (MODE == COLS_RHS) return;
@@ -562,11 +591,13 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   float2 W[PTS];
 #pragma unroll
   for (int q = 0; q < PTS; ++q) {
-    float2 w;
-    if constexpr (MODE == COLS_PRIME)
-      w = inband(q) ? ld2(bwc, vs, soff_s(q)) : make_float2(0.f, 0.f);
-    else
-      w = lds[swz(t + q * T)];
+    float2 w = make_float2(0.f, 0.f);
+    if (!qout(q)) {
+      if constexpr (MODE == COLS_PRIME)
+        w = inband(q) ? ld2(bwc, vs, soff_s(q)) : make_float2(0.f, 0.f);
+      else
+        w = lds[swz(t + q * T)];
+    }
     W[q] = cscale(w, S);
   }
 #pragma unroll 1
@@ -579,6 +610,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     float2 X[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
+      if (qout(q)) {
+        X[q] = make_float2(0.f, 0.f);
+        continue;
+      }
       const int idx = tl + q * T;
       const float2 w = W[q];
       const float fky = (float)wn(idx, N);

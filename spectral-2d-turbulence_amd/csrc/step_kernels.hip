@@ -94,6 +94,28 @@ struct ColCfg {
 // ---------------------------------------------------------------------------------------------
 // row kernel
 // ---------------------------------------------------------------------------------------------
+// Blocks of a line (thread t, block q holds element i = t + q T) that lie out of band for every t:
+// K < i < N - K for all of them, a run in the middle (q = 6..9 of 16 at N = 4096). A forward row
+// transform's outputs there are never read by the Hermitian unpack; the column kernel's mode loop
+// takes blocks in pairs and skips the pairs inside this run.
+template <int N>
+struct ColBand {
+  static constexpr int T = Line<N>::T, PTS = Line<N>::PTS, K = Geo<N>::K;
+  static constexpr bool out(int q) { return q * T > K && (q + 1) * T - 1 < N - K; }
+  static constexpr int skip_first() {
+    for (int q = 0; q + 1 < PTS; q += 2)
+      if (out(q) && out(q + 1)) return q;
+    return PTS;
+  }
+  static constexpr int skip_count() {
+    int n = 0;
+    for (int q = skip_first(); q + 1 < PTS && out(q) && out(q + 1); q += 2) n += 2;
+    return n;
+  }
+};
+static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() == 4,
+              "4096: blocks 6..9 are out of band");
+
 // Hermitian unpack along x of a forward row transform z held by the line (thread t holds
 // z[t + q T]): the two real fields packed as re / im, for kx = 0..K, into 2-row-tiled planes.
 // `voff` is the lane's byte offset of (row pair, kx = t). PAIR: the fields are rows 2m and 2m+1
@@ -107,7 +129,8 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
   using Ge = Geo<N>;
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < L::PTS; ++q) lds[swz(t + q * L::T)] = z[q];
+  for (int q = 0; q < L::PTS; ++q)
+    if (!ColBand<N>::out(q)) lds[swz(t + q * L::T)] = z[q];  // the others are never read
   __syncthreads();
   if (!live) return;
 #pragma unroll
@@ -379,26 +402,6 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
 // ---------------------------------------------------------------------------------------------
 // column kernel
 // ---------------------------------------------------------------------------------------------
-// Row blocks of a column line (thread t, block q holds row i = t + q T) that lie out of band for
-// every t: K < i < N - K for all of them, a run in the middle (q = 6..9 of 16 at N = 4096). The
-// column kernel's mode loop takes blocks in pairs and skips the pairs inside this run.
-template <int N>
-struct ColBand {
-  static constexpr int T = Line<N>::T, PTS = Line<N>::PTS, K = Geo<N>::K;
-  static constexpr bool out(int q) { return q * T > K && (q + 1) * T - 1 < N - K; }
-  static constexpr int skip_first() {
-    for (int q = 0; q + 1 < PTS; q += 2)
-      if (out(q) && out(q + 1)) return q;
-    return PTS;
-  }
-  static constexpr int skip_count() {
-    int n = 0;
-    for (int q = skip_first(); q + 1 < PTS && out(q) && out(q + 1); q += 2) n += 2;
-    return n;
-  }
-};
-static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() == 4,
-              "4096: blocks 6..9 are out of band");
 
 template <int N, int C, int MODE, bool FORCE>
 __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {

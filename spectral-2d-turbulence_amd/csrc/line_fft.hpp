@@ -17,6 +17,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
+#include "bufio.hpp"
+
 namespace s2t {
 
 // ---------------------------------------------------------------------------------------------
@@ -209,55 +213,82 @@ __device__ __forceinline__ void load_bases(float2 (&bs)[Bases<N>::SIZE], const f
           tw[L::TW2 + j * L::R0 * L::R1 + (t + i * L::T) % (L::R0 * L::R1)];
 }
 
-// Middle/last Stockham pass: NB butterflies of radix R per thread, span NS, inputs from LDS.
-// Twiddle bases from the table `tw`, or, for BOFF >= 0, from `cached` (this pass's bases start at
-// BOFF).
-template <int N, int R, int NS, int DIR, bool LAST, int BOFF = -1>
-__device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2* lds, int t,
-                                              const float2* __restrict__ tw,
-                                              const float2* cached) {
+
+// Twiddle bases of one Stockham pass (radix R, span NS) for this thread's NB butterflies.
+template <int N, int R, int NS>
+struct PassBases {
+  static constexpr int NB = N / (R * Line<N>::T);  // butterflies per thread
+  static constexpr int PER = TwBase<R>::NB;        // bases per butterfly
+  static constexpr int SIZE = NB * PER;
+};
+
+// Load this thread's bases of one pass from its table, which starts `off` values into the
+// buffer `tb` (pass table: base j of span offset k at j * NS + k). Buffer loads count in vmcnt
+// only; a generic (flat) load also counts in lgkmcnt, so the wait for an exchange's LDS writes
+// before its barrier would wait for the twiddle loads in flight as well.
+template <int N, int R, int NS, int S>
+__device__ __forceinline__ void load_pass_bases(float2 (&bs)[S], Buf tb, int off, int t) {
+  using P = PassBases<N, R, NS>;
+  static_assert(S == P::SIZE, "bases array of this pass");
+#pragma unroll
+  for (int i = 0; i < P::NB; ++i)
+#pragma unroll
+    for (int j = 0; j < P::PER; ++j)
+      bs[i * P::PER + j] = ld2(tb, (uint32_t)((t + i * Line<N>::T) % NS) * 8u,
+                               (uint32_t)(off + j * NS) * 8u);
+}
+
+// Middle/last Stockham pass, compute part: NB butterflies of radix R per thread, span NS, inputs
+// read from LDS, twiddled with the bases `bs` (NB x TwBase<R>::NB, see load_pass_bases), DFT in
+// registers. The results stay in u for pass_store.
+template <int N, int R, int NS, int DIR>
+__device__ __forceinline__ void pass_compute(float2 (&u)[N / (R * Line<N>::T)][R], const float2* lds,
+                                             int t, const float2* bs) {
   constexpr int T = Line<N>::T;
   constexpr int NB = N / (R * T);
   constexpr int STRIDE = N / R;
-  float2 u[NB][R];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int b = t + i * T;
-    const int k = b % NS;
 #pragma unroll
     for (int r = 0; r < R; ++r) u[i][r] = lds[swz(b + r * STRIDE)];
-    float2 w[R];  // w[r] = W_{NS R}^{r k}, r = 1..R-1
+    const float2* bi = bs + i * TwBase<R>::NB;
+    float2 w[R];  // w[r] = W_{NS R}^{r k}, r = 1..R-1, k = b mod NS
     if constexpr (R >= 8) {
-      float2 bs[TwBase<R>::NB];
-#pragma unroll
-      for (int j = 0; j < TwBase<R>::NB; ++j)
-        bs[j] = BOFF >= 0 ? cached[BOFF + i * TwBase<R>::NB + j] : tw[j * NS + k];
-      w[1] = bs[0];
-      w[2] = bs[1];
-      w[4] = bs[2];
-      w[3] = cmul(bs[1], bs[0]);
-      w[5] = cmul(bs[2], bs[0]);
-      w[6] = cmul(bs[2], bs[1]);
-      w[7] = cmul(bs[3], cconj(bs[0]));
+      w[1] = bi[0];
+      w[2] = bi[1];
+      w[4] = bi[2];
+      w[3] = cmul(bi[1], bi[0]);
+      w[5] = cmul(bi[2], bi[0]);
+      w[6] = cmul(bi[2], bi[1]);
+      w[7] = cmul(bi[3], cconj(bi[0]));
       if constexpr (R == 16) {
-        w[8] = bs[3];
-        w[9] = cmul(bs[3], bs[0]);
-        w[10] = cmul(bs[3], bs[1]);
-        w[11] = cmul(bs[3], w[3]);
-        w[12] = cmul(bs[3], bs[2]);
-        w[13] = cmul(bs[4], cconj(w[3]));
-        w[14] = cmul(bs[4], cconj(bs[1]));
-        w[15] = cmul(bs[4], cconj(bs[0]));
+        w[8] = bi[3];
+        w[9] = cmul(bi[3], bi[0]);
+        w[10] = cmul(bi[3], bi[1]);
+        w[11] = cmul(bi[3], w[3]);
+        w[12] = cmul(bi[3], bi[2]);
+        w[13] = cmul(bi[4], cconj(w[3]));
+        w[14] = cmul(bi[4], cconj(bi[1]));
+        w[15] = cmul(bi[4], cconj(bi[0]));
       }
     } else {
 #pragma unroll
-      for (int r = 1; r < R; ++r)
-        w[r] = BOFF >= 0 ? cached[BOFF + i * TwBase<R>::NB + r - 1] : tw[(r - 1) * NS + k];
+      for (int r = 1; r < R; ++r) w[r] = bi[r - 1];
     }
 #pragma unroll
     for (int r = 1; r < R; ++r) u[i][r] = cmul(u[i][r], DIR > 0 ? cconj(w[r]) : w[r]);
     dft<R, DIR>(u[i]);
   }
+}
+
+// Middle/last Stockham pass, store part: the last pass leaves X[t + q T] in v; a middle pass
+// writes its outputs to LDS in Stockham order for the next pass (barrier, writes, barrier).
+template <int N, int R, int NS, bool LAST>
+__device__ __forceinline__ void pass_store(float2 (&u)[N / (R * Line<N>::T)][R],
+                                           float2 (&v)[Line<N>::PTS], float2* lds, int t) {
+  constexpr int T = Line<N>::T;
+  constexpr int NB = N / (R * T);
   if constexpr (LAST) {
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -277,6 +308,36 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Line<N>::PTS], float2*
   }
 }
 
+// The passes of a line transform after the first DFT. Each pass's twiddle bases come from
+// `fetch(pass, array)` (pass = std::integral_constant 1 or 2; a table load or a copy of
+// registers), issued before the exchange that precedes the pass so that a table load's latency
+// overlaps the barrier and the LDS round trip.
+template <int N, int DIR, class Fetch>
+__device__ __forceinline__ void line_passes(float2 (&v)[Line<N>::PTS], float2* lds, int t,
+                                            Fetch&& fetch) {
+  using L = Line<N>;
+  using P1 = PassBases<N, L::R1, L::R0>;
+  float2 b1[P1::SIZE];
+  fetch(std::integral_constant<int, 1>{}, b1);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
+  __syncthreads();
+  float2 u1[P1::NB][L::R1];
+  pass_compute<N, L::R1, L::R0, DIR>(u1, lds, t, b1);
+  if constexpr (L::NPASS == 2) {
+    pass_store<N, L::R1, L::R0, true>(u1, v, lds, t);
+  } else {
+    using P2 = PassBases<N, L::R2, L::R0 * L::R1>;
+    float2 b2[P2::SIZE];
+    fetch(std::integral_constant<int, 2>{}, b2);
+    pass_store<N, L::R1, L::R0, false>(u1, v, lds, t);
+    float2 u2[P2::NB][L::R2];
+    pass_compute<N, L::R2, L::R0 * L::R1, DIR>(u2, lds, t, b2);
+    pass_store<N, L::R2, L::R0 * L::R1, true>(u2, v, lds, t);
+  }
+}
+
 // Full line transform. On entry v[q] = x[t + q*T]; on exit v[q] = X[t + q*T].
 // Every thread of the workgroup must call this the same number of times (it contains barriers);
 // threads whose line is unused still call it on whatever data they hold.
@@ -288,17 +349,14 @@ __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds,
   // out of the callers' field loops, which would pin ~60 VGPRs for the whole kernel.
   const float2* twl = tw;
   asm volatile("" : "+s"(twl));
+  const Buf tb = make_buf(twl, (uint32_t)(L::TWSIZE * 8));
   dft<L::R0, DIR>(v);
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
-  __syncthreads();
-  if constexpr (L::NPASS == 2) {
-    stockham_pass<N, L::R1, L::R0, DIR, true>(v, lds, t, twl + L::TW1, nullptr);
-  } else {
-    stockham_pass<N, L::R1, L::R0, DIR, false>(v, lds, t, twl + L::TW1, nullptr);
-    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true>(v, lds, t, twl + L::TW2, nullptr);
-  }
+  line_passes<N, DIR>(v, lds, t, [&](auto pass, auto& bs) {
+    if constexpr (decltype(pass)::value == 1)
+      load_pass_bases<N, L::R1, L::R0>(bs, tb, L::TW1, t);
+    else
+      load_pass_bases<N, L::R2, L::R0 * L::R1>(bs, tb, L::TW2, t);
+  });
 }
 
 // The same transform with the twiddle bases already in registers (load_bases).
@@ -307,26 +365,19 @@ __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds,
                                          const float2 (&bases)[Bases<N>::SIZE]) {
   using L = Line<N>;
   using B = Bases<N>;
-  // opaque copy of the bases: the twiddles derived from them stay local to this transform
-  // instead of being hoisted (all 30 of them) to the caller's scope
-  float2 bs[B::SIZE];
-#pragma unroll
-  for (int i = 0; i < B::SIZE; ++i) {
-    float x = bases[i].x, y = bases[i].y;
-    asm volatile("" : "+v"(x), "+v"(y));
-    bs[i] = make_float2(x, y);
-  }
   dft<L::R0, DIR>(v);
-  __syncthreads();
+  // opaque copies of the bases: the twiddles derived from them stay local to this transform
+  // instead of being hoisted (all 30 of them) to the caller's scope
+  line_passes<N, DIR>(v, lds, t, [&](auto pass, auto& bs) {
+    constexpr int n = sizeof(bs) / sizeof(float2);
+    constexpr int off = decltype(pass)::value == 1 ? 0 : B::NB1 * B::NT1;
 #pragma unroll
-  for (int r = 0; r < L::R0; ++r) lds[swz(t * L::R0 + r)] = v[r];
-  __syncthreads();
-  if constexpr (L::NPASS == 2) {
-    stockham_pass<N, L::R1, L::R0, DIR, true, 0>(v, lds, t, nullptr, bs);
-  } else {
-    stockham_pass<N, L::R1, L::R0, DIR, false, 0>(v, lds, t, nullptr, bs);
-    stockham_pass<N, L::R2, L::R0 * L::R1, DIR, true, B::NB1 * B::NT1>(v, lds, t, nullptr, bs);
-  }
+    for (int i = 0; i < n; ++i) {
+      float x = bases[off + i].x, y = bases[off + i].y;
+      asm volatile("" : "+v"(x), "+v"(y));
+      bs[i] = make_float2(x, y);
+    }
+  });
 }
 
 }  // namespace s2t

@@ -74,3 +74,23 @@ def test_stage0_forms_agree(s2t, n):
     err = np.max(np.abs(a - b)) / np.max(np.abs(b))
     print("n=%d stage-0 forms: speed %.9g vs %.9g, state rel diff %.2e" % (n, spa, spb, err))
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_steps_are_deterministic(s2t, n):
+    """The same state stepped twice (stochastic forcing on, adaptive dt) gives bit-identical
+    states and clocks: every reduction in the step is order-independent (max, fixed-order sums),
+    so any difference means a race or a hazard in a kernel."""
+    params = dict(beta=10.0, drag=0.01, viscosity=1.0, hyper_p=4.0, force_amp=4.0, force_k=32.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02)
+    runs = []
+    for _ in range(2):
+        with s2t.Solver(n) as s:
+            s.configure(**params)
+            s.seed_spectrum(k0=n / 64, amplitude=6.0, seed=21)
+            s.step(3)
+            d = s.diagnostics()
+            runs.append((s.read_state(), d["dt"], d["time"], d["maxSpeed"]))
+    (a, *ca), (b, *cb) = runs
+    assert ca == cb
+    assert np.array_equal(a, b)

@@ -305,9 +305,10 @@ int s2t_rhs(s2t_solver* h, uint32_t step_index, double dt, float* rhs) {
       hipchk(hipMalloc(&s->force_k, s->ops->state_elems * sizeof(float2)), "force spectrum");
   }
   ensure_primed(s);
-  s->ops->rows(s, 1, false, false, force);
+  // the planes of a primed (or stepped) state are in the stage-0 form
+  const bool sd = s->ops->rows(s, 0, false, false, force);
   float2* compact = s->full + 3 * (size_t)s->n * s->n;
-  s->ops->rhs(s, force, compact, step_index, dt);
+  s->ops->rhs(s, force, sd, compact, step_index, dt);
   s->ops->unpack(s, compact, s->full);
   hipchk(hipMemcpyAsync(rhs, s->full, (size_t)s->n * s->n * sizeof(float2), hipMemcpyDeviceToHost,
                         s->stream),

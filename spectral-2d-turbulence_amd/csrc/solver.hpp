@@ -24,7 +24,7 @@ struct Ops {
   bool (*rows)(Solver* s, int stage, bool speed, bool vel, bool force);
   void (*cols)(Solver* s, int stage, bool force, bool sd);
   void (*prime)(Solver* s, int i0, int ni);
-  void (*rhs)(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt);
+  void (*rhs)(Solver* s, bool force, bool sd, float2* rhs_out, uint32_t step, double dt);
   // full-field helpers
   void (*fft2)(Solver* s, float2* data, int inverse);
   void (*pack)(Solver* s, const float2* full, float2* state);

@@ -63,7 +63,7 @@ struct ColArgs {
   size_t s_mz, s_mf, s_mu, s_state;  // per-instance strides (elements)
   int stage;
   int force;
-  int sd;  // the products arrive as S', D' (sum / difference form, ROWS_PAIRED_SD)
+  int sd;  // the products arrive as S', D' (sum / difference form: the stage-0 row kernels)
   uint32_t step_override;
   double dt_override;
 };
@@ -158,9 +158,7 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
 // (row pair, kx = T - t) in a 2-row-tiled plane. VPAIR: x = a(y0) + i a(y0+1) for one real field
 // a (plane offset soffa; one 16-byte load per value); otherwise x = a(y) + i b(y) for two real
 // fields of one row (uniform offsets soffa / soffb include the row's 8-byte slot).
-//
-// COMB = +1 / -1 (with VPAIR): the real field is a +- b of two planes (offsets soffa, soffb).
-template <int N, bool VPAIR, int COMB = 0>
+template <int N, bool VPAIR>
 __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int t, bool live,
                                          uint32_t vlo, uint32_t vhi, uint32_t soffa,
                                          uint32_t soffb) {
@@ -173,12 +171,6 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
   auto pair = [&](uint32_t v, uint32_t so, float2& va, float2& vb) {
     if constexpr (VPAIR) {
       ld4(mu, v, soffa + so, va, vb);
-      if constexpr (COMB != 0) {
-        float2 wa, wb;
-        ld4(mu, v, soffb + so, wa, wb);
-        va = COMB > 0 ? cadd(va, wa) : csub(va, wa);
-        vb = COMB > 0 ? cadd(vb, wb) : csub(vb, wb);
-      }
     } else {
       va = ld2(mu, v, soffa + so);
       vb = ld2(mu, v, soffb + so);
@@ -205,18 +197,21 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
   }
 }
 
-// Row kernel variants. ROWS_SINGLE pairs the transforms per row (B' = v'(y0) + i v'(y1) once per
-// row pair, A' = omega' + i u' per row) so that u and v of a row are in registers together, as the
-// CFL reduction and the dye velocity need (stage 0); ROWS_SINGLE_FORCE adds the pointer forcing.
+// Row kernel variants. The stage-0 planes hold omega', s' = u' + v' and d' = u' - v' (the column
+// kernel writes that form at stage 2 and in the prime); after stages 0 and 1 they hold omega', u'
+// and v'. Stage 0 needs the CFL maximum of |u| + |v| = max(|s|, |d|), and it stores the products
+// s omega, d omega as S', D' (the column kernel recovers N_hat = i kx P_hat + i ky Q_hat from them
+// by linearity).
+// ROWS_PAIRED_SD (stage 0) transforms the omega, s and d pairs of two rows at once, like
+// ROWS_PAIRED; the maximum is max(max|s|, max|d|), each from one transform.
+// ROWS_SINGLE (stage 0 with dye) pairs the transforms per row (B' = d'(y0) + i d'(y1) once per row
+// pair, A' = omega' + i s' per row) so that s and d of a row are in registers together, for the dye
+// velocity u = (s + d) / 2, v = (s - d) / 2; ROWS_SINGLE_FORCE adds the pointer forcing.
 // ROWS_PAIRED (stages 1 and 2) transforms omega, u and v each for the two rows of a pair at once
 // and packs the products across the pair (u0 omega0 + i u1 omega1, v0 omega0 + i v1 omega1), so
-// every load and store is one 16-byte row-pair element; both variants do five line FFTs per pair.
-// ROWS_PAIRED_SD (stage 0 without dye or pointer forcing) transforms s = u + v and d = u - v in
-// place of u and v: |u| + |v| = max(|s|, |d|), so the CFL maximum is max(max|s|, max|d|), each
-// available from one transform, and the products s omega, d omega are stored as S', D' (the column
-// kernel recovers N_hat = i kx P_hat + i ky Q_hat from them by linearity). Same five FFTs and
-// 16-byte accesses as ROWS_PAIRED. (Holding the u pair through the v transform instead needs
-// ~210 VGPRs; staging it in a scratch row costs 134 MB of HBM traffic per launch.)
+// every load is one 16-byte row-pair element. Every variant does five line FFTs per row pair.
+// (The stage-0 maximum with u', v' planes needed the u pair held through the v transform, ~210
+// VGPRs, or s and d each formed from both planes, a second read of u' and v'.)
 enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2, ROWS_PAIRED_SD = 3 };
 
 template <int N, int G, int KIND>
@@ -254,8 +249,8 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   float* vpark = vpark_all + g * N;
   // Twiddle bases of this thread, loaded once for all transforms of the kernel: only where the
   // LDS limit (three workgroups per CU at N = 4096) leaves the registers free, in the paired
-  // variant (146 VGPRs with them; the stage-0 variants would exceed 168, smaller N would lose
-  // occupancy).
+  // variant of stages 1-2 (138 VGPRs with them; the stage-0 variants reach 170 and more, past the
+  // 168 of three waves per SIMD; smaller N would lose occupancy).
   constexpr bool TWREG = KIND == ROWS_PAIRED && N == 4096;
   float2 tb[Bases<N>::SIZE];
   if constexpr (TWREG) load_bases<N>(tb, a.tw, t);
@@ -289,20 +284,13 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
         vpark[tp + q * T] = X[q].y;
       }
       // f = 0: u pair -> P'(y0), P'(y1); f = 1: v pair -> Q'(y0), Q'(y1)
-      // (SD: f = 0: s = u + v -> S', f = 1: d = u - v -> D', with the CFL maximum of |s|, |d|)
+      // (SD: f = 0: s pair -> S', f = 1: d pair -> D', with the CFL maximum of |s|, |d|)
 #pragma unroll 1
       for (int f = 0; f < 2; ++f) {
         const int tr = launder(tp);
         const uint32_t vlr = Ge::off(y0, tr);
         const uint32_t vhr = Ge::off(y0, T - tr);
-        if constexpr (SD) {
-          if (f == 0)
-            row_load<N, true, +1>(X, bu, tr, live, vlr, vhr, PLANE_B, 2 * PLANE_B);
-          else
-            row_load<N, true, -1>(X, bu, tr, live, vlr, vhr, PLANE_B, 2 * PLANE_B);
-        } else {
-          row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
-        }
+        row_load<N, true>(X, bu, tr, live, vlr, vhr, (uint32_t)(1 + f) * PLANE_B, 0u);
         fft_inv(X, tr);
         if constexpr (SD) {
 #pragma unroll
@@ -327,14 +315,14 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
     // lane byte offsets of (row pair y0 / 2, kx = t) and (row pair, kx = T - t) in a plane
     const uint32_t vlo = Ge::off(y0, tp);
     const uint32_t vhi = Ge::off(y0, T - tp);
-    // v of rows y0, y0+1 from one complex transform: B' = V0 + i V1, Hermitian-extended
+    // d of rows y0, y0+1 from one complex transform: B' = D0 + i D1, Hermitian-extended
     float2 B[PTS];
     row_load<N, true>(B, bu, tp, live, vlo, vhi, 2 * PLANE_B, 0u);
     fft_inv(B, tp);
-    float v0[PTS];
+    float d0[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
-      v0[q] = B[q].x;
+      d0[q] = B[q].x;
       vpark[tp + q * T] = B[q].y;
     }
 
@@ -344,23 +332,24 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
       const int y = y0 + r;
       const uint32_t vlr = Ge::off(y0, tr);
       const uint32_t vhr = Ge::off(y0, T - tr);
-      // A' = omega' + i u' of row y (its 8-byte slot r in the row pair)
+      // A' = omega' + i s' of row y (its 8-byte slot r in the row pair)
       float2 A[PTS];
       row_load<N, false>(A, bu, tr, live, vlr, vhr, (uint32_t)r * 8u, PLANE_B + (uint32_t)r * 8u);
       fft_inv(A, tr);
-      // physical omega = Re A, u = Im A, v = Re/Im B; packed product Z = u omega + i v omega
+      // physical omega = Re A, s = Im A, d = Re/Im B; packed product Z = s omega + i d omega
       const uint32_t vvel = ((uint32_t)y * N + (uint32_t)tr) * 8u;
 #pragma unroll
       for (int q = 0; q < PTS; ++q) {
-        const float om = A[q].x, u = A[q].y;
-        const float v = r == 0 ? v0[q] : vpark[tr + q * T];
+        const float om = A[q].x, sv = A[q].y;
+        const float dv = r == 0 ? d0[q] : vpark[tr + q * T];
         if (a.flags & ROW_SPEED) {
-          const float sp = fabsf(u) + fabsf(v);
+          const float sp = fmaxf(fabsf(sv), fabsf(dv));  // = |u| + |v|
           bad |= !(sp <= 3.4e38f);
           smax = fmaxf(smax, sp);
         }
-        if (do_vel && live) st2(bv, make_float2(u, v), vvel, (uint32_t)(q * T) * 8u);
-        A[q] = make_float2(u * om, v * om);
+        if (do_vel && live)
+          st2(bv, make_float2(0.5f * (sv + dv), 0.5f * (sv - dv)), vvel, (uint32_t)(q * T) * 8u);
+        A[q] = make_float2(sv * om, dv * om);
       }
       fft_fwd(A, tr);
       row_unpack_store<N, false>(A, lds, tr, live, bz, vlr, (uint32_t)r * 8u,
@@ -603,6 +592,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     }
     W[q] = cscale(w, S);
   }
+  // After stage 2 and in the prime the next row kernel is a stage-0 one: it takes s' = u' + v'
+  // and d' = u' - v' (s_hat = i (ky - kx) w_hat / k^2, d_hat = i (ky + kx) w_hat / k^2) in the
+  // planes of u' and v'.
+  const bool sdo = MODE == COLS_PRIME || a.stage == 2;
 #pragma unroll 1
   for (int f = 0; f < 3; ++f) {
     // re-derive the lane's indices each field so nothing per-value stays live across transforms
@@ -610,6 +603,9 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     const int kxl = launder(kx);
     const uint32_t vml = Ge::off(tl, kxl);
     const float fkxl = (float)kxl;
+    // field f = 1, 2: g = (cy ky + cx kx) / k^2, the transform input i g w_hat
+    const float cy = f == 1 || sdo ? 1.f : 0.f;
+    const float cx = f == 1 ? (sdo ? -1.f : 0.f) : (sdo ? 1.f : -1.f);
     float2 X[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
@@ -624,7 +620,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       // hardware reciprocal (1 ulp; the IEEE quotient takes a dozen instructions per value)
       const float inv = k2 > 0.f ? __builtin_amdgcn_rcpf(k2) : 0.f;
       // u_hat = i ky / k^2 omega_hat, v_hat = -i kx / k^2 omega_hat
-      const float g = (f == 1 ? fky : -fkxl) * inv;
+      const float g = (cy * fky + cx * fkxl) * inv;
       X[q] = f == 0 ? w : cscale(cmuli(w), g);
     }
     line_fft<N, +1>(X, lds, tl, a.tw);
@@ -762,7 +758,8 @@ struct Impl {
     if constexpr (L::NPASS == 3) put_pass<L::R2>(h, L::TW2, L::R0 * L::R1);
   }
 
-  // returns true when the products were written in the sum / difference form (S', D')
+  // Returns true when the products were written in the sum / difference form (S', D'): the
+  // stage-0 kinds, which read the omega', s', d' planes. vel and force apply at stage 0 only.
   static bool rows(Solver* s, int stage, bool speed, bool vel, bool force) {
     RowArgs a{};
     a.mu = s->mixed_u;
@@ -780,17 +777,18 @@ struct Impl {
     a.flags = (speed ? ROW_SPEED : 0) | (vel ? ROW_VEL : 0) | (force ? ROW_FORCE : 0);
     const int grid = (RowCfg<N>::NB + 7) / 8 * 8;
     prof_begin(s, S2T_PROF_ROWS);
-    bool sd = false;
-    if (force) {
+    const bool sd = stage == 0;
+    if (!sd && (speed || vel || force))
+      throw std::logic_error("row kernel: speed, dye velocity and pointer forcing are stage-0 work");
+    if (sd && force) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE_FORCE>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
-    } else if (vel) {
+    } else if (sd && vel) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
-    } else if (speed) {
+    } else if (sd) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_PAIRED_SD>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
-      sd = true;
     } else {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_PAIRED>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
@@ -853,10 +851,11 @@ struct Impl {
   }
 
   // fused right-hand side of instance 0 with an explicit step index and dt (test hook)
-  static void rhs(Solver* s, bool force, float2* rhs_out, uint32_t step, double dt) {
+  static void rhs(Solver* s, bool force, bool sd, float2* rhs_out, uint32_t step, double dt) {
     ColArgs a = col_args(s, 0);
     a.rhs_out = rhs_out;
     a.force = force ? 1 : 0;
+    a.sd = sd ? 1 : 0;
     a.step_override = step;
     a.dt_override = dt;
     constexpr int GROUPS = (Ge::KC + C - 1) / C;

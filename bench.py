@@ -239,7 +239,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: a single instance does not shard across GPUs "
-              "in this build (see DESIGN.md); use --gpus 1", file=sys.stderr)
+              "in this build; use --gpus 1", file=sys.stderr)
         return 2
 
     s2t = load_binding()

@@ -181,7 +181,7 @@ int s2t_rhs(s2t_solver* h, uint32_t step_index, double dt, float* rhs);
 
 /* ------------------------------------------------------------------------------------------ */
 /* batched ensembles: many independent instances of mixed grid sizes stepped together           */
-/* (no counterpart in the reference, which runs one solver; SURVEY next-component f1)            */
+/* (no counterpart in the reference, which runs one solver)                                     */
 /* ------------------------------------------------------------------------------------------ */
 
 typedef struct s2t_batch s2t_batch;
@@ -205,7 +205,7 @@ int s2t_batch_step(s2t_batch* b, int nsteps);
 int s2t_batch_sync(s2t_batch* b);
 
 /* ------------------------------------------------------------------------------------------ */
-/* serving: continuous batching of simulation requests (SURVEY next-component f2)               */
+/* serving: continuous batching of simulation requests (no counterpart in the reference)        */
 /* ------------------------------------------------------------------------------------------ */
 
 typedef struct s2t_server s2t_server;

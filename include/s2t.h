@@ -115,6 +115,16 @@ int s2t_seed_spectrum(s2t_solver* h, double k0, double amplitude, uint32_t seed)
 /* Solver.seedPhysical(values) (solver.js:361-382): n*n physical vorticity -> forward FFT ->
  * 2/3 mask, k = 0 zeroed (src/shaders/seed.wgsl:4-17). Also resets like Solver.reset. */
 int s2t_seed_physical(s2t_solver* h, const float* field);
+/* Band-limited random vorticity (no counterpart in the reference; the initial condition of the
+ * 1024^2 decaying-turbulence case): every in-band mode (2/3 rule, k != 0) with
+ * kmin <= |k| <= kmax gets a standard-normal amplitude and a uniform phase, drawn per canonical
+ * mode (ky > 0, or ky = 0 and kx > 0) from hash32 streams of the seed (src/gpu/solver.js:43-47);
+ * the mirror mode is the conjugate, and the field is scaled to rms vorticity `amplitude`.
+ * s2t_bandlimited_spectrum fills n*n interleaved floats on the host (no GPU needed);
+ * s2t_seed_bandlimited stores the same spectrum as the state and resets like s2t_seed_spectrum. */
+int s2t_bandlimited_spectrum(int n, double kmin, double kmax, double amplitude, uint32_t seed,
+                             float* spectrum);
+int s2t_seed_bandlimited(s2t_solver* h, double kmin, double kmax, double amplitude, uint32_t seed);
 /* Write the vorticity spectrum (2*n*n floats, fixture layout). The field must be the spectrum of
  * a real field; it is stored masked to the 2/3 band with k = 0 zeroed. Step index, time and dt
  * are left as they are (the reference harness writes state[0] directly, tests/index.html:187). */
@@ -221,6 +231,7 @@ typedef struct s2t_server s2t_server;
 #define S2T_REQ_RUNNING 1
 #define S2T_REQ_DONE 2
 #define S2T_REQ_REJECTED 3
+#define S2T_REQ_CANCELLED 4
 
 typedef struct s2t_request {
   int n;                 /* grid size; must be one of the server's classes */
@@ -234,7 +245,7 @@ typedef struct s2t_request {
 } s2t_request;
 
 typedef struct s2t_server_stats {
-  uint64_t submitted, completed, rejected;
+  uint64_t submitted, completed, rejected, cancelled;
   double p50_latency_s, p99_latency_s, mean_latency_s;
   double mpts_steps_per_s; /* completed work / time from first submit to last completion */
   double budget_bytes, used_bytes;
@@ -244,7 +255,8 @@ typedef struct s2t_server_stats {
  * checked against budget_bytes (0 = no limit). A worker thread admits requests as slots free. */
 int s2t_server_create(const int* sizes, const int* slots, int nclasses, double budget_bytes,
                       s2t_server** out);
-/* Stops the worker after the requests in flight; frees everything. */
+/* Cancels the requests still queued or running (their callbacks run), stops the worker and
+ * frees everything. */
 int s2t_server_destroy(s2t_server* s);
 /* Thread-safe. A size without a class is rejected (status S2T_REQ_REJECTED). */
 int s2t_server_submit(s2t_server* s, const s2t_request* r, uint64_t* id);
@@ -252,7 +264,17 @@ int s2t_server_submit(s2t_server* s, const s2t_request* r, uint64_t* id);
 int s2t_server_poll(s2t_server* s, uint64_t id, int* status, s2t_diag* diag, double* latency_s);
 /* Final spectrum (2*n*n floats, fixture layout) of a finished request submitted with keep_state. */
 int s2t_server_read_state(s2t_server* s, uint64_t id, float* spectrum);
-/* Wait until every submitted request is done or rejected (timeout_s <= 0: no limit). */
+/* Thread-safe. A queued request is dropped at once; a running one is stopped by the worker
+ * before its next steps (its slot is freed, no result is kept). Status S2T_REQ_CANCELLED.
+ * S2T_ESTATE if the request has already finished (done, rejected or cancelled). */
+int s2t_server_cancel(s2t_server* s, uint64_t id);
+/* Completion callback, called once per request when it reaches a final status (done, rejected,
+ * cancelled): on the worker thread, or on the submitting / cancelling thread when the status is
+ * final at that call. It runs without the server's lock held, so it may call s2t_server_poll and
+ * s2t_server_read_state; it must not call s2t_server_destroy. NULL removes it. */
+typedef void (*s2t_server_callback)(uint64_t id, int status, void* user);
+int s2t_server_set_callback(s2t_server* s, s2t_server_callback cb, void* user);
+/* Wait until every submitted request is done, rejected or cancelled (timeout_s <= 0: no limit). */
 int s2t_server_drain(s2t_server* s, double timeout_s);
 int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);
 

@@ -186,6 +186,42 @@ def fill_spectrum(n, k0=14.0, amplitude=6.0, seed=1):
     return re.astype(np.complex64) + 1j * im.astype(np.complex64)
 
 
+def bandlimited_spectrum(n, kmin=8.0, kmax=16.0, amplitude=6.0, seed=2):
+    """Band-limited random vorticity of s2t_bandlimited_spectrum, returned as complex64 N x N.
+
+    This build's initial condition for the decaying-turbulence case (no counterpart in the
+    reference, which seeds with fillSpectrum). Every in-band mode (2/3 rule, k != 0) with
+    kmin <= |k| <= kmax is drawn per canonical mode (ky > 0, or ky = 0 and kx > 0; idx = row*n +
+    col of the canonical mode): u_j = (hash32((3 idx + j) ^ hash32(seed)) + 1/2) / 2^32,
+    g = sqrt(-2 ln u_0) cos(2 pi u_1) (Box-Muller), phase 2 pi u_2, value g e^{i phase} (the mirror
+    takes the conjugate); FP64 throughout, scaled so that the physical field has rms `amplitude`
+    (sqrt(sum g^2) / N^2), rounded once to float32.
+
+    Pinned by: Hermitian symmetry, support, rms normalisation (Parseval), k = 0 zero, the
+    Gaussian ratio E|g| / sqrt(E g^2) = sqrt(2/pi), and hash known answers.
+    """
+    g = Grid(n)
+    kr = np.sqrt(g.k2)
+    sel = g.band & (g.k2 > 0) & (kr >= kmin) & (kr <= kmax)
+    rows, cols = np.indices((n, n))
+    canon = (g.ky > 0) | ((g.ky == 0) & (g.kx > 0))
+    hr = np.where(canon, rows, (n - rows) % n)
+    hc = np.where(canon, cols, (n - cols) % n)
+    idx = (hr * n + hc).astype(np.uint32)
+    salt = hash1(np.uint32(seed & 0xFFFFFFFF))
+    u = [(hash1((np.uint32(3) * idx + np.uint32(j)) ^ salt).astype(np.float64) + 0.5)
+         / 4294967296.0 for j in range(3)]
+    amp = np.sqrt(-2.0 * np.log(u[0])) * np.cos(TWO_PI * u[1])
+    ph = TWO_PI * u[2]
+    sgn = np.where(canon, 1.0, -1.0)
+    re = np.where(sel, amp * np.cos(ph), 0.0)
+    im = np.where(sel, sgn * amp * np.sin(ph), 0.0)
+    total = float(np.sum(np.where(sel, amp * amp, 0.0)))
+    rms = np.sqrt(total) / (float(n) * n)
+    scale = amplitude / rms if rms > 0 else 0.0
+    return (re * scale).astype(np.float32) + 1j * (im * scale).astype(np.float32)
+
+
 def _grid(n, fn):
     h = TWO_PI / n
     y, x = np.meshgrid(np.arange(n) * h, np.arange(n) * h, indexing="ij")

@@ -32,7 +32,8 @@ MAX_SPLATS = 8
 # every symbol include/s2t.h declares
 EXPORTS = [
     "s2t_create", "s2t_destroy", "s2t_last_error", "s2t_radix_schedule", "s2t_configure",
-    "s2t_nu_h", "s2t_set_splats", "s2t_seed_spectrum", "s2t_seed_physical", "s2t_write_state",
+    "s2t_nu_h", "s2t_set_splats", "s2t_seed_spectrum", "s2t_seed_physical",
+    "s2t_bandlimited_spectrum", "s2t_seed_bandlimited", "s2t_write_state",
     "s2t_read_state", "s2t_reset_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
@@ -42,10 +43,11 @@ EXPORTS = [
     "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
     "s2t_batch_sync",
     "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
-    "s2t_server_read_state", "s2t_server_drain", "s2t_server_get_stats",
+    "s2t_server_read_state", "s2t_server_cancel", "s2t_server_set_callback", "s2t_server_drain",
+    "s2t_server_get_stats",
 ]
 SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
-REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED = 0, 1, 2, 3
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED = 0, 1, 2, 3, 4
 
 
 class Params(ctypes.Structure):
@@ -74,9 +76,14 @@ class Request(ctypes.Structure):
                 ("params", Params)]
 
 
+# void (*)(uint64_t id, int status, void* user)
+SERVER_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p)
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
-                ("rejected", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double),
+                ("rejected", ctypes.c_uint64), ("cancelled", ctypes.c_uint64),
+                ("p50_latency_s", ctypes.c_double),
                 ("p99_latency_s", ctypes.c_double), ("mean_latency_s", ctypes.c_double),
                 ("mpts_steps_per_s", ctypes.c_double), ("budget_bytes", ctypes.c_double),
                 ("used_bytes", ctypes.c_double)]
@@ -112,6 +119,8 @@ def load(path=None):
         "s2t_set_splats": (ip, [vp, ctypes.POINTER(SplatC), ip]),
         "s2t_seed_spectrum": (ip, [vp, dp, dp, u32]),
         "s2t_seed_physical": (ip, [vp, fp]),
+        "s2t_bandlimited_spectrum": (ip, [ip, dp, dp, dp, u32, fp]),
+        "s2t_seed_bandlimited": (ip, [vp, dp, dp, dp, u32]),
         "s2t_write_state": (ip, [vp, fp]),
         "s2t_read_state": (ip, [vp, fp]),
         "s2t_reset_clock": (ip, [vp]),
@@ -152,6 +161,8 @@ def load(path=None):
         "s2t_server_poll": (ip, [vp, ctypes.c_uint64, ctypes.POINTER(ip), ctypes.POINTER(Diag),
                                  ctypes.POINTER(dp)]),
         "s2t_server_read_state": (ip, [vp, ctypes.c_uint64, fp]),
+        "s2t_server_cancel": (ip, [vp, ctypes.c_uint64]),
+        "s2t_server_set_callback": (ip, [vp, SERVER_CALLBACK, vp]),
         "s2t_server_drain": (ip, [vp, dp]),
         "s2t_server_get_stats": (ip, [vp, ctypes.POINTER(ServerStats)]),
     }
@@ -252,6 +263,11 @@ class Solver:
         f = np.ascontiguousarray(np.asarray(field, np.float32).reshape(-1))
         assert f.size == self.n * self.n
         _check(self.lib, self.lib.s2t_seed_physical(self.h, _ptr(f)))
+
+    def seed_bandlimited(self, kmin=8.0, kmax=16.0, amplitude=6.0, seed=2):
+        """s2t_seed_bandlimited: random in-band modes with kmin <= |k| <= kmax, rms amplitude."""
+        _check(self.lib, self.lib.s2t_seed_bandlimited(self.h, float(kmin), float(kmax),
+                                                       float(amplitude), int(seed) & 0xFFFFFFFF))
 
     def write_state(self, w):
         a = to_interleaved(np.asarray(w).reshape(self.n, self.n))
@@ -382,6 +398,15 @@ def _diag_dict(d, spec=None, zonal=None):
     return out
 
 
+def bandlimited_spectrum(n, kmin=8.0, kmax=16.0, amplitude=6.0, seed=2):
+    """s2t_bandlimited_spectrum (host only): the band-limited random spectrum as complex64 n x n."""
+    lib = load()
+    a = np.empty(2 * n * n, np.float32)
+    _check(lib, lib.s2t_bandlimited_spectrum(int(n), float(kmin), float(kmax), float(amplitude),
+                                             int(seed) & 0xFFFFFFFF, _ptr(a)))
+    return from_interleaved(a, n)
+
+
 class Batch:
     """Ensemble of independent members of mixed grid sizes (s2t_batch_*)."""
 
@@ -508,6 +533,17 @@ class Server:
         a = np.empty(2 * n * n, np.float32)
         _check(self.lib, self.lib.s2t_server_read_state(self.h, ctypes.c_uint64(rid), _ptr(a)))
         return from_interleaved(a, n)
+
+    def cancel(self, rid):
+        """s2t_server_cancel: drop a queued request, or stop a running one between steps."""
+        _check(self.lib, self.lib.s2t_server_cancel(self.h, ctypes.c_uint64(rid)))
+
+    def set_callback(self, fn):
+        """fn(id, status) once per request at its final status, on the server's worker thread (or
+        on the submitting / cancelling thread when final at that call); None removes it."""
+        self._cb = SERVER_CALLBACK(lambda rid, status, user: fn(int(rid), int(status))) \
+            if fn is not None else SERVER_CALLBACK()
+        _check(self.lib, self.lib.s2t_server_set_callback(self.h, self._cb, None))
 
     def drain(self, timeout_s=0.0):
         _check(self.lib, self.lib.s2t_server_drain(self.h, float(timeout_s)))

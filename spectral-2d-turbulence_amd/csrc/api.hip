@@ -126,6 +126,36 @@ int s2t_seed_physical(s2t_solver* h, const float* field) {
   S2T_API_END
 }
 
+int s2t_bandlimited_spectrum(int n, double kmin, double kmax, double amplitude, uint32_t seed,
+                             float* spectrum) {
+  if (!spectrum) return api_bad("spectrum is NULL");
+  const std::string e = size_error(n);
+  if (!e.empty()) return api_bad(e);
+  if (!(kmin >= 0.0) || !(kmax >= kmin)) return api_bad("need 0 <= kmin <= kmax");
+  if (!(amplitude >= 0.0)) return api_bad("amplitude must be non-negative");
+  try {
+    bandlimited_spectrum(n, kmin, kmax, amplitude, seed, spectrum);
+  } catch (const std::bad_alloc&) {
+    g_err = "s2t_bandlimited_spectrum: out of host memory";
+    return S2T_ENOMEM;
+  }
+  return S2T_OK;
+}
+
+int s2t_seed_bandlimited(s2t_solver* h, double kmin, double kmax, double amplitude,
+                         uint32_t seed) {
+  SOLVER(h);
+  if (!(kmin >= 0.0) || !(kmax >= kmin)) return api_bad("need 0 <= kmin <= kmax");
+  if (!(amplitude >= 0.0)) return api_bad("amplitude must be non-negative");
+  S2T_API_BEGIN
+  std::vector<float> spec((size_t)2 * s->n * s->n);
+  bandlimited_spectrum(s->n, kmin, kmax, amplitude, seed, spec.data());
+  write_state_slot(s, 0, spec.data());
+  reset_slot(s, 0);
+  sync(s);
+  S2T_API_END
+}
+
 int s2t_write_state(s2t_solver* h, const float* spectrum) {
   SOLVER(h);
   if (!spectrum) return api_bad("spectrum is NULL");

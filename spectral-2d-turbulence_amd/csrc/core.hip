@@ -366,4 +366,48 @@ void step_once(Solver* s, bool dye) {
   }
 }
 
+
+// Band-limited random vorticity (host, FP64, rounded once to FP32). Per canonical mode with
+// kmin <= |k| <= kmax in the 2/3 band: three uniforms u_j = (hash32((3 idx + j) ^ hash32(seed))
+// + 1/2) / 2^32, idx = row * n + col of the canonical mode; g = sqrt(-2 ln u0) cos(2 pi u1)
+// (Box-Muller), phase 2 pi u2, value g (cos, sgn sin) with sgn = -1 on the mirror; then every
+// value is scaled by amplitude / (sqrt(sum g^2) / n^2), the rms of the physical field.
+void bandlimited_spectrum(int n, double kmin, double kmax, double amplitude, uint32_t seed,
+                          float* out) {
+  const double kcut = n / 3.0;
+  const uint32_t salt = hash1(seed);
+  const double two_pi = 6.283185307179586476925286766559;
+  const size_t nn = (size_t)n * n;
+  std::vector<double> re(nn, 0.0), im(nn, 0.0);
+  double total = 0.0;
+  for (int row = 0; row < n; ++row) {
+    const int ky = row <= n / 2 ? row : row - n;
+    for (int col = 0; col < n; ++col) {
+      const int kx = col <= n / 2 ? col : col - n;
+      if (!(std::fabs((double)kx) < kcut && std::fabs((double)ky) < kcut)) continue;
+      const double k2 = (double)kx * kx + (double)ky * ky;
+      const double kr = std::sqrt(k2);
+      if (k2 == 0.0 || kr < kmin || kr > kmax) continue;
+      const bool canon = ky > 0 || (ky == 0 && kx > 0);
+      const int hr = canon ? row : (n - row) % n, hc = canon ? col : (n - col) % n;
+      const uint32_t idx = (uint32_t)hr * (uint32_t)n + (uint32_t)hc;
+      double u[3];
+      for (int j = 0; j < 3; ++j)
+        u[j] = ((double)hash1((3u * idx + (uint32_t)j) ^ salt) + 0.5) / 4294967296.0;
+      const double g = std::sqrt(-2.0 * std::log(u[0])) * std::cos(two_pi * u[1]);
+      const double ph = two_pi * u[2];
+      const size_t i = (size_t)row * n + col;
+      re[i] = g * std::cos(ph);
+      im[i] = (canon ? 1.0 : -1.0) * g * std::sin(ph);
+      total += g * g;
+    }
+  }
+  const double rms = std::sqrt(total) / ((double)n * n);
+  const double scale = rms > 0.0 ? amplitude / rms : 0.0;
+  for (size_t i = 0; i < nn; ++i) {
+    out[2 * i] = (float)(re[i] * scale);
+    out[2 * i + 1] = (float)(im[i] * scale);
+  }
+}
+
 }  // namespace s2t

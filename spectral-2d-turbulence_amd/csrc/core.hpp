@@ -29,6 +29,9 @@ void solver_free(Solver* s);
 
 void configure_slot(Solver* s, int slot, const s2t_params& p);
 void reset_slot(Solver* s, int slot);
+// band-limited random spectrum (s2t_bandlimited_spectrum), n*n interleaved floats, host only
+void bandlimited_spectrum(int n, double kmin, double kmax, double amplitude, uint32_t seed,
+                          float* out);
 void seed_spectrum_slot(Solver* s, int slot, double k0, double amplitude, uint32_t seed);
 void seed_physical_slot(Solver* s, int slot, const float* field);
 void write_state_slot(Solver* s, int slot, const float* spectrum);

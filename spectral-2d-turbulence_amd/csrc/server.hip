@@ -93,6 +93,7 @@ struct Request {
 
 struct Record {
   int status = S2T_REQ_QUEUED;
+  bool cancel = false;  // asked to stop while running; the worker frees the slot
   int n = 0, steps = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
   s2t_diag diag{};
@@ -125,9 +126,23 @@ struct Server {
   std::vector<double> latencies;
   double work_pts_steps = 0;      // sum n^2 * steps of completed requests
   double t_first = 0, t_last = 0;
-  uint64_t submitted = 0, completed = 0, rejected = 0;
+  uint64_t submitted = 0, completed = 0, rejected = 0, cancelled = 0;
   std::string fatal;
+  s2t_server_callback callback = nullptr;
+  void* callback_user = nullptr;
 };
+
+// Completion callback (called without sv->mu held).
+static void notify(Server* sv, uint64_t id, int status) {
+  s2t_server_callback cb;
+  void* user;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    cb = sv->callback;
+    user = sv->callback_user;
+  }
+  if (cb) cb(id, status, user);
+}
 
 static size_t class_bytes(int n, int slots) {
   // device bytes of a class of `slots` instances of size n (solver_new's allocations)
@@ -195,17 +210,52 @@ static void complete(Server* sv, Class& c, int slot) {
   sv->t_last = t;
 }
 
+// Stop a running request whose cancellation was asked for: its slot goes idle on the device
+// (no step kernel of the class is in flight when the worker calls this) and is freed.
+static void cancel_slot(Server* sv, Class& c, int slot) {
+  const uint64_t id = (uint64_t)c.slot_req[slot];
+  const int zero = 0;
+  hipchk(hipMemcpy(&c.s->inst[slot].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
+         "hipMemcpy");
+  c.slot_req[slot] = -1;
+  c.active -= 1;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  Record& rc = sv->rec[id];
+  rc.status = S2T_REQ_CANCELLED;
+  rc.t_done = now_s();
+  sv->cancelled += 1;
+}
+
 static void worker_loop(Server* sv) {
   try {
     while (true) {
       // 1. admission: FIFO per grid size, into free slots
       std::vector<std::pair<int, Request>> take;
+      std::vector<uint64_t> dropped;  // queued requests cancelled by the shutdown
       {
         std::unique_lock<std::mutex> lk(sv->mu);
+        if (sv->stop) {
+          // shutdown: nothing more is admitted, and running requests stop between steps
+          for (const Request& r : sv->pending) {
+            Record& rc = sv->rec[r.id];
+            rc.status = S2T_REQ_CANCELLED;
+            rc.t_done = now_s();
+            sv->cancelled += 1;
+            dropped.push_back(r.id);
+          }
+          sv->pending.clear();
+          for (auto& kv : sv->cls)
+            for (int64_t id : kv.second.slot_req)
+              if (id >= 0) sv->rec[(uint64_t)id].cancel = true;
+        }
         bool any_active = false;
         for (auto& kv : sv->cls) any_active |= kv.second.active > 0;
         if (!any_active && sv->pending.empty()) {
-          if (sv->stop) break;
+          if (sv->stop) {
+            lk.unlock();
+            for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
+            break;
+          }
           sv->cv.wait_for(lk, std::chrono::milliseconds(2));
           continue;
         }
@@ -226,12 +276,31 @@ static void worker_loop(Server* sv) {
           }
         }
       }
+      for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
       for (auto& tr : take) {
         Class& c = sv->cls[tr.first];
         int slot = 0;
         while (c.slot_req[slot] >= 0) ++slot;
         admit(sv, c, slot, tr.second);
       }
+      // cancellations of running requests, between steps (every class stream is idle here)
+      std::vector<uint64_t> stopped;
+      for (auto& kv : sv->cls) {
+        Class& c = kv.second;
+        for (int slot = 0; slot < (int)c.slot_req.size(); ++slot) {
+          if (c.slot_req[slot] < 0) continue;
+          bool want;
+          {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            want = sv->rec[(uint64_t)c.slot_req[slot]].cancel;
+          }
+          if (want) {
+            stopped.push_back((uint64_t)c.slot_req[slot]);
+            cancel_slot(sv, c, slot);
+          }
+        }
+      }
+      for (uint64_t id : stopped) notify(sv, id, S2T_REQ_CANCELLED);
       // 2. step every class with active members, each on its own stream
       std::vector<std::pair<Class*, int>> ran;
       for (auto& kv : sv->cls) {
@@ -254,7 +323,11 @@ static void worker_loop(Server* sv) {
         for (int slot = 0; slot < c.s->batch; ++slot) {
           if (c.slot_req[slot] < 0) continue;
           const DevState d = read_clock_slot(c.s, slot);
-          if (!d.active) complete(sv, c, slot);
+          if (!d.active) {
+            const uint64_t id = (uint64_t)c.slot_req[slot];
+            complete(sv, c, slot);
+            notify(sv, id, S2T_REQ_DONE);
+          }
         }
       }
     }
@@ -360,7 +433,7 @@ int s2t_server_submit(s2t_server* h, const s2t_request* r, uint64_t* id) {
   if (r->scenario < 0 || r->scenario > S2T_SCENARIO_MERGER) return api_bad("unknown scenario");
   const std::string e = check_params(r->params);
   if (!e.empty()) return api_bad(e);
-  std::lock_guard<std::mutex> lk(sv->mu);
+  std::unique_lock<std::mutex> lk(sv->mu);
   if (!sv->fatal.empty()) return api_bad("server stopped: " + sv->fatal);
   const uint64_t rid = sv->next_id++;
   Record& rc = sv->rec[rid];
@@ -370,7 +443,8 @@ int s2t_server_submit(s2t_server* h, const s2t_request* r, uint64_t* id) {
   rc.keep_state = r->keep_state != 0;
   sv->submitted += 1;
   if (sv->t_first == 0) sv->t_first = rc.t_submit;
-  if (!sv->cls.count(r->n)) {
+  const bool reject = !sv->cls.count(r->n);
+  if (reject) {
     rc.status = S2T_REQ_REJECTED;
     sv->rejected += 1;
   } else {
@@ -378,6 +452,47 @@ int s2t_server_submit(s2t_server* h, const s2t_request* r, uint64_t* id) {
     sv->cv.notify_all();
   }
   *id = rid;
+  if (reject) {
+    lk.unlock();
+    notify(sv, rid, S2T_REQ_REJECTED);
+  }
+  return S2T_OK;
+}
+
+int s2t_server_cancel(s2t_server* h, uint64_t id) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv) return api_bad("s2t_server_cancel: null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->rec.find(id);
+  if (it == sv->rec.end()) return api_bad("unknown request id");
+  Record& rc = it->second;
+  if (rc.status == S2T_REQ_QUEUED) {
+    for (auto p = sv->pending.begin(); p != sv->pending.end(); ++p)
+      if (p->id == id) {
+        sv->pending.erase(p);
+        break;
+      }
+    rc.status = S2T_REQ_CANCELLED;
+    rc.t_done = now_s();
+    sv->cancelled += 1;
+    lk.unlock();
+    notify(sv, id, S2T_REQ_CANCELLED);
+    return S2T_OK;
+  }
+  if (rc.status == S2T_REQ_RUNNING) {
+    rc.cancel = true;  // the worker stops it between steps and calls back
+    return S2T_OK;
+  }
+  g_err = "request already finished";
+  return S2T_ESTATE;
+}
+
+int s2t_server_set_callback(s2t_server* h, s2t_server_callback cb, void* user) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv) return api_bad("s2t_server_set_callback: null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->callback = cb;
+  sv->callback_user = user;
   return S2T_OK;
 }
 
@@ -415,7 +530,7 @@ int s2t_server_drain(s2t_server* h, double timeout_s) {
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       if (!sv->fatal.empty()) return api_bad("server stopped: " + sv->fatal);
-      if (sv->completed + sv->rejected == sv->submitted) return S2T_OK;
+      if (sv->completed + sv->rejected + sv->cancelled == sv->submitted) return S2T_OK;
     }
     if (timeout_s > 0 && now_s() - t0 > timeout_s) {
       g_err = "drain timed out";
@@ -441,6 +556,7 @@ int s2t_server_get_stats(s2t_server* h, s2t_server_stats* out) {
   out->submitted = sv->submitted;
   out->completed = sv->completed;
   out->rejected = sv->rejected;
+  out->cancelled = sv->cancelled;
   out->p50_latency_s = pct(0.5);
   out->p99_latency_s = pct(0.99);
   out->mean_latency_s = l.empty() ? 0.0 : [&] { double t = 0; for (double x : l) t += x; return t / l.size(); }();

@@ -1,8 +1,10 @@
-"""The C ABI library loads and exports every entry point of include/s2t.h (no GPU calls)."""
+"""The C ABI library loads and exports every entry point of include/s2t.h (no GPU calls; the
+host-only spectrum generator is called)."""
 
 import os
 import re
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -47,3 +49,19 @@ def test_null_handles_rejected(s2t):
     lib = s2t.load()
     assert lib.s2t_step(None, 1, 0) == s2t.S2T_EINVAL
     assert lib.s2t_destroy(None) == 0
+
+
+@pytest.mark.parametrize("n, kmin, kmax", [(128, 4.0, 12.0), (1024, 8.0, 16.0)])
+def test_bandlimited_spectrum_host_matches_oracle(s2t, n, kmin, kmax):
+    """s2t_bandlimited_spectrum is host code (no GPU): the library's spectrum equals the oracle's
+    to one FP32 ulp (the FP64 normalisation sums run in different orders)."""
+    from oracle import gpusem as gs
+    got = s2t.bandlimited_spectrum(n, kmin, kmax, 6.0, seed=2)
+    want = gs.bandlimited_spectrum(n, kmin, kmax, 6.0, seed=2)
+    assert np.array_equal(got != 0, want != 0)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) <= np.finfo(np.float32).eps * scale
+    with pytest.raises(s2t.S2TError, match="4\\^a"):
+        s2t.bandlimited_spectrum(100, 8.0, 16.0)
+    with pytest.raises(s2t.S2TError, match="kmin"):
+        s2t.bandlimited_spectrum(128, 16.0, 8.0)

@@ -94,3 +94,20 @@ def test_steps_are_deterministic(s2t, n):
     (a, *ca), (b, *cb) = runs
     assert ca == cb
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [128, 1024])
+def test_seed_bandlimited_stores_the_host_spectrum(s2t, n):
+    """s2t_seed_bandlimited stores exactly s2t_bandlimited_spectrum (every mode is in band) and
+    resets the clock like s2t_seed_spectrum."""
+    want = s2t.bandlimited_spectrum(n, 8.0, 16.0, 6.0, seed=2)
+    with s2t.Solver(n) as s:
+        s.configure(**PHYS)
+        s.seed_spectrum(k0=8.0, amplitude=5.0, seed=1)
+        s.step(2)
+        s.seed_bandlimited(8.0, 16.0, 6.0, seed=2)
+        assert np.array_equal(s.read_state(), want)
+        d = s.diagnostics()
+        assert d["stepIndex"] == 0 and d["time"] == 0.0
+        s.step(1)
+        assert s.diagnostics()["stepIndex"] == 1

@@ -71,3 +71,48 @@ def test_queueing_beyond_slots(s2t):
         for rid in ids:
             st, d, lat = sv.poll(rid)
             assert st == s2t.REQ_DONE and d["stepIndex"] >= 10 and lat > 0
+
+
+def test_cancel_and_completion_callbacks(s2t):
+    """A queued request is dropped at once and a running one is stopped between steps, its slot
+    going to the next request (whose result is still the plain solver's, bit for bit); every
+    request reports its final status exactly once through the callback; a finished request
+    cannot be cancelled."""
+    import threading
+    seen, lock = {}, threading.Lock()
+
+    def on_final(rid, status):
+        with lock:
+            assert rid not in seen
+            seen[rid] = status
+
+    seed = dict(scenario="turbulence", k0=6.0, amplitude=5.0, seed=11)
+    with s2t.Server({64: 1}) as sv:
+        sv.set_callback(on_final)
+        long_id = sv.submit(n=64, steps=10 ** 7, **seed, **PHYS)   # runs until cancelled
+        queued = sv.submit(n=64, steps=10, **seed, **PHYS)        # waits for the one slot
+        rejected = sv.submit(n=128, steps=5, **PHYS)              # no class of that size
+        t0 = time.time()
+        while sv.poll(long_id)[0] != s2t.REQ_RUNNING:
+            assert time.time() - t0 < 60
+            time.sleep(0.005)
+        sv.cancel(queued)
+        assert sv.poll(queued)[0] == s2t.REQ_CANCELLED
+        after = sv.submit(n=64, steps=7, keep_state=True, **seed, **PHYS)
+        sv.cancel(long_id)
+        sv.drain(120)
+        assert sv.poll(long_id)[0] == s2t.REQ_CANCELLED
+        st, d, lat = sv.poll(after)
+        assert st == s2t.REQ_DONE and d["stepIndex"] == 7 and lat > 0
+        got = sv.read_state(after)
+        with pytest.raises(s2t.S2TError, match="already finished"):
+            sv.cancel(after)
+        stats = sv.stats()
+    assert seen == {long_id: s2t.REQ_CANCELLED, queued: s2t.REQ_CANCELLED,
+                    rejected: s2t.REQ_REJECTED, after: s2t.REQ_DONE}
+    assert (stats["cancelled"], stats["completed"], stats["rejected"]) == (2, 1, 1)
+    with s2t.Solver(64) as s:
+        s.configure(**PHYS)
+        s.seed_spectrum(k0=6.0, amplitude=5.0, seed=11)
+        s.step(7)
+        assert np.array_equal(got, s.read_state())

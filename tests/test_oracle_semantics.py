@@ -160,3 +160,37 @@ def test_dye_invariants():
     out = gs.dye_step(d, vel, 0.01).astype(float)
     assert out[..., :3].min() >= d[..., :3].astype(float).min() - 1e-3
     assert out[..., :3].max() <= d[..., :3].astype(float).max() + 1e-3
+
+
+# ---------------------------------------------------------------------------------------------
+# band-limited random seed (s2t_bandlimited_spectrum)
+# ---------------------------------------------------------------------------------------------
+def test_bandlimited_spectrum_invariants():
+    n, kmin, kmax, amp = 256, 8.0, 16.0, 6.0
+    w = gs.bandlimited_spectrum(n, kmin, kmax, amp, seed=2).astype(np.complex128)
+    g = Grid(n)
+    kr = np.sqrt(g.k2)
+    support = g.band & (g.k2 > 0) & (kr >= kmin) & (kr <= kmax)
+    assert np.all(w[~support] == 0) and np.all(w[support] != 0)
+    assert w[0, 0] == 0
+    # Hermitian: w(-k) = conj w(k), so the physical field is real
+    assert np.array_equal(w[g.mirror][:, g.mirror], np.conj(w))
+    phys = np.fft.ifft2(w)
+    assert np.max(np.abs(phys.imag)) < 1e-12 * np.max(np.abs(phys.real))
+    assert abs(np.sqrt(np.mean(phys.real ** 2)) / amp - 1) < 1e-6
+    # determinism and seed dependence
+    assert np.array_equal(gs.bandlimited_spectrum(n, kmin, kmax, amp, seed=2), w.astype(np.complex64))
+    assert not np.array_equal(gs.bandlimited_spectrum(n, kmin, kmax, amp, seed=3), w.astype(np.complex64))
+
+
+def test_bandlimited_amplitudes_are_gaussian():
+    """|value| = |g| for a standard normal g: E|g| / sqrt(E g^2) = sqrt(2/pi) = 0.798 (a uniform
+    amplitude would give 0.866, a constant one 1). About 2,500 canonical modes here."""
+    n = 256
+    w = gs.bandlimited_spectrum(n, 0.0, 40.0, 1.0, seed=7)
+    g = Grid(n)
+    canon = ((g.ky > 0) | ((g.ky == 0) & (g.kx > 0))) & (w != 0)
+    a = np.abs(w[canon]).astype(np.float64)
+    assert a.size > 2000
+    ratio = np.mean(a) / np.sqrt(np.mean(a * a))
+    assert abs(ratio - np.sqrt(2 / np.pi)) < 0.02

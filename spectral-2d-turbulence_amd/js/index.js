@@ -303,8 +303,94 @@ const SCENARIOS = {
 
 const SCENARIO_NAMES = Object.keys(SCENARIOS);
 
+// request statuses and server-side scenarios of include/s2t.h
+const REQ = Object.freeze({ QUEUED: 0, RUNNING: 1, DONE: 2, REJECTED: 3, CANCELLED: 4 });
+const SERVER_SCENARIOS = Object.freeze({ turbulence: 0, 'taylor-green': 1, dipole: 2, merger: 3 });
+
+// Serving loop (include/s2t.h s2t_server_*): requests (a scenario, a grid size and a step count) share pre-allocated solver
+// slots per grid size and run on the library's worker thread. submit() returns the request id at
+// once and a Promise that settles when the request reaches its final status: resolved with the
+// FP64 diagnostics (and the final spectrum with keepState) when done, rejected with an Error
+// whose `status` is REQ.REJECTED or REQ.CANCELLED otherwise.
+class Server {
+  // classes: [{n, slots}], budgetBytes: device memory limit for the slots (0 = none)
+  constructor(classes, { budgetBytes = 0 } = {}) {
+    this.pending = new Map();
+    this.destroyed = false;
+    this.handle = addon().serverCreate(
+      classes.map((c) => c.n), classes.map((c) => c.slots), budgetBytes,
+      (id, status, result) => this._final(id, status, result));
+    addon().serverRef(this.handle, false);
+  }
+
+  // request: {n, steps, scenario ('turbulence' | 'taylor-green' | 'dipole' | 'merger'),
+  //           k0, amplitude, seed, mergerSep, keepState}; config: solver keys (DEFAULT_CONFIG)
+  submit(request, config = {}) {
+    const scen = SERVER_SCENARIOS[request.scenario || 'turbulence'];
+    if (scen === undefined) throw new Error(`unknown server scenario ${request.scenario}`);
+    const req = {
+      ...request,
+      scenario: scen,
+      scenarioParam: request.mergerSep !== undefined ? request.mergerSep : 0.7,
+      keepState: request.keepState ? 1 : 0,
+    };
+    let settle;
+    const result = new Promise((resolve, reject) => { settle = { resolve, reject }; });
+    const first = this.pending.size === 0;
+    if (first) addon().serverRef(this.handle, true);
+    let id;
+    try {
+      // a rejection's callback is queued during submit and runs after this returns
+      id = addon().serverSubmit(this.handle, req, { ...DEFAULT_CONFIG, ...config });
+    } catch (e) {
+      if (first) addon().serverRef(this.handle, false);
+      throw e;
+    }
+    this.pending.set(id, settle);
+    return { id, result };
+  }
+
+  poll(id) {
+    return addon().serverPoll(this.handle, id);
+  }
+
+  cancel(id) {
+    addon().serverCancel(this.handle, id);
+  }
+
+  stats() {
+    return addon().serverStats(this.handle);
+  }
+
+  // result: diagnostics (and `state`) read by the library's thread when the request finished
+  _final(id, status, result) {
+    const s = this.pending.get(id);
+    if (!s) return;
+    this.pending.delete(id);
+    if (this.pending.size === 0 && !this.destroyed) addon().serverRef(this.handle, false);
+    if (status === REQ.DONE) {
+      s.resolve(result);
+    } else {
+      const e = new Error(status === REQ.CANCELLED ? `request ${id} cancelled`
+        : `request ${id} rejected`);
+      e.status = status;
+      s.reject(e);
+    }
+  }
+
+  // cancels what is still queued or running (their Promises are rejected with REQ.CANCELLED) and
+  // frees the slots; call it when done, as the completion function holds this object alive
+  destroy() {
+    if (this.destroyed) return;
+    this.destroyed = true;
+    addon().serverDestroy(this.handle);
+  }
+}
+
 module.exports = {
   Solver,
+  Server,
+  REQ,
   Fft2D,
   SCENARIOS,
   SCENARIO_NAMES,

@@ -2,12 +2,15 @@
 // C entry point and throws a JS Error carrying s2t_last_error() on a negative status (the
 // reference throws Error for programming errors, src/gpu/fft.js:28,84). Long device work that
 // returns data (diagnostics) is also offered as a Promise on a worker thread, like the reference's
-// async readDiagnostics (src/gpu/solver.js:570-608).
+// async readDiagnostics (src/gpu/solver.js:570-608). The serving loop reports each request's final
+// status from its native worker thread through a thread-safe function.
 #include <node_api.h>
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "s2t.h"
@@ -138,13 +141,8 @@ napi_value RadixSchedule(napi_env env, napi_callback_info info) {
   return obj;
 }
 
-// configure(h, cfg): the reference config keys (src/main.js:9-36, solver.js:306-324)
-napi_value Configure(napi_env env, napi_callback_info info) {
-  Args a;
-  if (!get_args(env, info, a, 2)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
-  if (!h) return nullptr;
-  napi_value c = a.argv[1];
+// the reference config keys (src/main.js:9-36, solver.js:306-324) with their defaults
+s2t_params params_of(napi_env env, napi_value c) {
   s2t_params p{};
   p.beta = prop(env, c, "beta", 0);
   p.drag = prop(env, c, "drag", 0.01);
@@ -158,6 +156,16 @@ napi_value Configure(napi_env env, napi_callback_info info) {
   p.dt_max = prop(env, c, "dtMax", 0.02);
   p.dye_fade = prop(env, c, "dyeFade", 0.999);
   p.fixed_dt = prop(env, c, "fixedDt", 0);
+  return p;
+}
+
+// configure(h, cfg): the reference config keys (src/main.js:9-36, solver.js:306-324)
+napi_value Configure(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  s2t_solver* h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  const s2t_params p = params_of(env, a.argv[1]);
   S2T_CALL(env, s2t_configure(h, &p));
   return nullptr;
 }
@@ -497,6 +505,287 @@ napi_value Rhs(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// ---------------------------------------------------------------------------------------------
+// serving loop (s2t_server_*)
+// ---------------------------------------------------------------------------------------------
+struct ServerBox {
+  s2t_server* sv = nullptr;
+  napi_threadsafe_function tsfn = nullptr;
+  // grid size of each outstanding request submitted with keepState; recursive because a
+  // rejection calls back on the submitting thread inside s2t_server_submit
+  std::recursive_mutex mu;
+  std::unordered_map<uint64_t, int> keep;
+};
+
+// a request's final status with its result, read on the thread that reports it, so that the
+// result outlives a destroy() that comes before the JS thread runs the callback
+struct Final {
+  uint64_t id;
+  int status;
+  s2t_diag diag{};
+  double latency = -1;
+  std::vector<float> state;
+};
+
+// s2t_server_callback: runs on the server's worker thread (or on the submitting / cancelling
+// thread when the status is final at that call) without the server's lock, so it may poll
+void server_final(uint64_t id, int status, void* user) {
+  ServerBox* b = static_cast<ServerBox*>(user);
+  Final* f = new Final{id, status};
+  int n = 0;
+  {
+    std::lock_guard<std::recursive_mutex> lk(b->mu);
+    auto it = b->keep.find(id);
+    if (it != b->keep.end()) {
+      n = it->second;
+      b->keep.erase(it);
+    }
+  }
+  if (status == S2T_REQ_DONE) {
+    int st = 0;
+    s2t_server_poll(b->sv, id, &st, &f->diag, &f->latency);
+    if (n > 0) {
+      f->state.resize((size_t)2 * n * n);
+      if (s2t_server_read_state(b->sv, id, f->state.data()) != S2T_OK) f->state.clear();
+    }
+  }
+  if (napi_call_threadsafe_function(b->tsfn, f, napi_tsfn_nonblocking) != napi_ok) delete f;
+}
+
+napi_value diag_object(napi_env env, int status, const s2t_diag& d, double latency) {
+  napi_value obj;
+  napi_create_object(env, &obj);
+  napi_set_named_property(env, obj, "status", make_num(env, status));
+  napi_set_named_property(env, obj, "latency", make_num(env, latency));
+  napi_set_named_property(env, obj, "energy", make_num(env, d.energy));
+  napi_set_named_property(env, obj, "enstrophy", make_num(env, d.enstrophy));
+  napi_set_named_property(env, obj, "dt", make_num(env, d.dt));
+  napi_set_named_property(env, obj, "maxSpeed", make_num(env, d.max_speed));
+  napi_set_named_property(env, obj, "time", make_num(env, d.time));
+  napi_set_named_property(env, obj, "stepIndex", make_num(env, d.step_index));
+  return obj;
+}
+
+// calls onFinal(id, status, result): result is the diagnostics object (with `state`, a
+// Float32Array in fixture layout, for keepState) when done, else undefined
+void final_to_js(napi_env env, napi_value js_cb, void*, void* data) {
+  Final* f = static_cast<Final*>(data);
+  if (env && js_cb) {
+    napi_value args[3], undef;
+    napi_get_undefined(env, &undef);
+    napi_create_double(env, (double)f->id, &args[0]);
+    napi_create_int32(env, f->status, &args[1]);
+    args[2] = undef;
+    if (f->status == S2T_REQ_DONE) {
+      args[2] = diag_object(env, f->status, f->diag, f->latency);
+      if (!f->state.empty()) {
+        void* p = nullptr;
+        napi_value arr = new_typed(env, napi_float32_array, f->state.size(), 4, &p);
+        if (arr) {
+          memcpy(p, f->state.data(), f->state.size() * sizeof(float));
+          napi_set_named_property(env, args[2], "state", arr);
+        }
+      }
+    }
+    napi_call_function(env, undef, js_cb, 3, args, nullptr);
+  }
+  delete f;
+}
+
+void server_close(ServerBox* b) {
+  if (b->sv) {
+    s2t_server_destroy(b->sv);  // cancels what is left; its callbacks are queued first
+    b->sv = nullptr;
+  }
+  if (b->tsfn) {
+    napi_release_threadsafe_function(b->tsfn, napi_tsfn_release);
+    b->tsfn = nullptr;
+  }
+}
+
+void finalize_server(napi_env, void* data, void*) {
+  ServerBox* b = static_cast<ServerBox*>(data);
+  server_close(b);
+  delete b;
+}
+
+ServerBox* server_box(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p ||
+      !static_cast<ServerBox*>(p)->sv) {
+    napi_throw_type_error(env, "S2T_HANDLE", "not a server handle (destroyed?)");
+    return nullptr;
+  }
+  return static_cast<ServerBox*>(p);
+}
+
+bool int_array(napi_env env, napi_value v, std::vector<int>& out) {
+  bool is = false;
+  napi_is_array(env, v, &is);
+  if (!is) {
+    napi_throw_type_error(env, "S2T_ARGS", "expected an array of numbers");
+    return false;
+  }
+  uint32_t len = 0;
+  napi_get_array_length(env, v, &len);
+  out.resize(len);
+  for (uint32_t i = 0; i < len; ++i) {
+    napi_value e;
+    napi_get_element(env, v, i, &e);
+    out[i] = (int)num(env, e, 0);
+  }
+  return true;
+}
+
+// serverCreate(sizes[], slots[], budgetBytes, onFinal(id, status)) -> handle
+napi_value ServerCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 4)) return nullptr;
+  std::vector<int> sizes, slots;
+  if (!int_array(env, a.argv[0], sizes) || !int_array(env, a.argv[1], slots)) return nullptr;
+  if (sizes.size() != slots.size()) {
+    napi_throw_type_error(env, "S2T_ARGS", "sizes and slots differ in length");
+    return nullptr;
+  }
+  ServerBox* b = new ServerBox();
+  const int rc = s2t_server_create(sizes.data(), slots.data(), (int)sizes.size(),
+                                   num(env, a.argv[2], 0), &b->sv);
+  if (rc != S2T_OK) {
+    delete b;
+    return throw_status(env, rc);
+  }
+  napi_value name;
+  napi_create_string_utf8(env, "s2t_server_final", NAPI_AUTO_LENGTH, &name);
+  if (napi_create_threadsafe_function(env, a.argv[3], nullptr, name, 0, 1, nullptr, nullptr,
+                                      nullptr, final_to_js, &b->tsfn) != napi_ok) {
+    server_close(b);
+    delete b;
+    napi_throw_error(env, "S2T_NAPI", "napi_create_threadsafe_function failed");
+    return nullptr;
+  }
+  s2t_server_set_callback(b->sv, server_final, b);
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, b, finalize_server, nullptr, &ext));
+  return ext;
+}
+
+// serverSubmit(h, {n, steps, scenario, scenarioParam, k0, amplitude, seed, keepState}, cfg) -> id
+napi_value ServerSubmit(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  napi_value r = a.argv[1];
+  s2t_request q{};
+  q.n = (int)prop(env, r, "n", 0);
+  q.steps = (int)prop(env, r, "steps", 0);
+  q.scenario = (int)prop(env, r, "scenario", S2T_SCENARIO_TURBULENCE);
+  q.scenario_param = prop(env, r, "scenarioParam", 0);
+  q.k0 = prop(env, r, "k0", 14);
+  q.amplitude = prop(env, r, "amplitude", 6);
+  q.seed = (uint32_t)prop(env, r, "seed", 1);
+  q.keep_state = prop(env, r, "keepState", 0) != 0;
+  q.params = params_of(env, a.argv[2]);
+  uint64_t id = 0;
+  int rc;
+  {
+    std::lock_guard<std::recursive_mutex> lk(b->mu);
+    rc = s2t_server_submit(b->sv, &q, &id);
+    // a request rejected inside the call has already called back; one admitted waits on mu
+    if (rc == S2T_OK && q.keep_state) {
+      int st = S2T_REQ_QUEUED;
+      s2t_server_poll(b->sv, id, &st, nullptr, nullptr);
+      if (st != S2T_REQ_REJECTED) b->keep[id] = q.n;
+    }
+  }
+  S2T_CALL(env, rc);
+  return make_num(env, (double)id);
+}
+
+// serverPoll(h, id) -> {status, latency, diagnostics fields}
+napi_value ServerPoll(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  int status = 0;
+  s2t_diag d{};
+  double lat = -1;
+  S2T_CALL(env, s2t_server_poll(b->sv, (uint64_t)num(env, a.argv[1], 0), &status, &d, &lat));
+  return diag_object(env, status, d, lat);
+}
+
+napi_value ServerCancel(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  S2T_CALL(env, s2t_server_cancel(b->sv, (uint64_t)num(env, a.argv[1], 0)));
+  return nullptr;
+}
+
+// serverReadState(h, id, n) -> Float32Array(2 n^2), fixture layout
+napi_value ServerReadState(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  const size_t n = (size_t)num(env, a.argv[2], 0);
+  void* data = nullptr;
+  napi_value arr = new_typed(env, napi_float32_array, 2 * n * n, 4, &data);
+  if (!arr) return nullptr;
+  S2T_CALL(env, s2t_server_read_state(b->sv, (uint64_t)num(env, a.argv[1], 0),
+                                      static_cast<float*>(data)));
+  return arr;
+}
+
+napi_value ServerStats(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  s2t_server_stats st{};
+  S2T_CALL(env, s2t_server_get_stats(b->sv, &st));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  napi_set_named_property(env, obj, "submitted", make_num(env, (double)st.submitted));
+  napi_set_named_property(env, obj, "completed", make_num(env, (double)st.completed));
+  napi_set_named_property(env, obj, "rejected", make_num(env, (double)st.rejected));
+  napi_set_named_property(env, obj, "cancelled", make_num(env, (double)st.cancelled));
+  napi_set_named_property(env, obj, "p50Latency", make_num(env, st.p50_latency_s));
+  napi_set_named_property(env, obj, "p99Latency", make_num(env, st.p99_latency_s));
+  napi_set_named_property(env, obj, "meanLatency", make_num(env, st.mean_latency_s));
+  napi_set_named_property(env, obj, "mptsStepsPerS", make_num(env, st.mpts_steps_per_s));
+  return obj;
+}
+
+// serverRef(h, on): whether the pending completion callbacks keep the event loop alive (the JS
+// class holds it while requests are outstanding)
+napi_value ServerRef(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  bool on = false;
+  napi_get_value_bool(env, a.argv[1], &on);
+  if (on)
+    NAPI_CALL(env, napi_ref_threadsafe_function(env, b->tsfn));
+  else
+    NAPI_CALL(env, napi_unref_threadsafe_function(env, b->tsfn));
+  return nullptr;
+}
+
+// serverDestroy(h): cancels what is left (those callbacks still arrive), stops the worker, and
+// releases the thread-safe function so the event loop may end
+napi_value ServerDestroy(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  void* p = nullptr;
+  if (napi_get_value_external(env, a.argv[0], &p) == napi_ok && p)
+    server_close(static_cast<ServerBox*>(p));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct Fn {
     const char* name;
@@ -510,7 +799,11 @@ napi_value Init(napi_env env, napi_value exports) {
              {"diagnostics", Diagnostics}, {"diagnosticsAsync", DiagnosticsAsync},
              {"paintDye", PaintDye},     {"readDye", ReadDye},
              {"fft2", Fft2},             {"velocityPack", VelocityPack},
-             {"assemble", Assemble},     {"rhs", Rhs}};
+             {"assemble", Assemble},     {"rhs", Rhs},
+             {"serverCreate", ServerCreate}, {"serverSubmit", ServerSubmit},
+             {"serverPoll", ServerPoll}, {"serverCancel", ServerCancel},
+             {"serverReadState", ServerReadState}, {"serverStats", ServerStats},
+             {"serverRef", ServerRef}, {"serverDestroy", ServerDestroy}};
   for (const Fn& f : fns) {
     napi_value fn;
     NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

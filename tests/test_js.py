@@ -54,3 +54,72 @@ def test_js_harness_fixtures():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL CHECKS PASS" in r.stdout
+
+
+SERVER_JS = r"""
+const fs = require('fs');
+const m = require('./index.js');
+const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };
+const seed = { scenario: 'turbulence', k0: 6, amplitude: 5, seed: 11 };
+const settle = (p) => p.then((r) => ({ ok: r }), (e) => ({ err: e.message, status: e.status }));
+(async () => {
+  const sv = new m.Server([{ n: 64, slots: 1 }]);
+  const long = sv.submit({ n: 64, steps: 1e7, ...seed }, phys);       // runs until cancelled
+  const queued = sv.submit({ n: 64, steps: 10, ...seed }, phys);      // waits for the one slot
+  const rejected = sv.submit({ n: 128, steps: 5, ...seed }, phys);    // no class of that size
+  const t0 = Date.now();
+  while (sv.poll(long.id).status !== m.REQ.RUNNING) {
+    if (Date.now() - t0 > 60000) throw new Error('never admitted');
+    await new Promise((r) => setTimeout(r, 5));
+  }
+  sv.cancel(queued.id);
+  const done = sv.submit({ n: 64, steps: 7, keepState: true, ...seed }, phys);
+  sv.cancel(long.id);
+  const out = {
+    long: await settle(long.result), queued: await settle(queued.result),
+    rejected: await settle(rejected.result), done: await settle(done.result),
+  };
+  fs.writeFileSync(process.argv[1], Buffer.from(out.done.ok.state.buffer));
+  delete out.done.ok.state;
+  // destroy() with a request still running rejects its Promise as cancelled
+  const last = sv.submit({ n: 64, steps: 1e7, ...seed }, phys);
+  while (sv.poll(last.id).status !== m.REQ.RUNNING) await new Promise((r) => setTimeout(r, 5));
+  out.stats = sv.stats();
+  sv.destroy();
+  out.last = await settle(last.result);
+  console.log(JSON.stringify(out));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_server_promises(s2t, tmp_path):
+    """The N-API serving loop: each submit's Promise settles once, from the library's worker
+    thread through a thread-safe function: done with the FP64 diagnostics and, with keepState,
+    the final spectrum, which equals the plain solver's bit for bit; rejected (no class of that
+    size) and cancelled (queued, running, or outstanding at destroy) reject with the status.
+    The event loop ends by itself once nothing is outstanding."""
+    import json
+
+    import numpy as np
+    state = tmp_path / "state.f32"
+    r = subprocess.run([NODE, "-e", SERVER_JS, str(state)], cwd=JS, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["long"]["status"] == 4 and out["long"]["err"].endswith("cancelled")
+    assert out["queued"]["status"] == 4 and out["rejected"]["status"] == 3
+    assert out["last"]["status"] == 4
+    d = out["done"]["ok"]
+    assert d["status"] == 2 and d["stepIndex"] == 7 and d["latency"] > 0
+    st = out["stats"]
+    assert (st["submitted"], st["completed"], st["rejected"], st["cancelled"]) == (5, 1, 1, 2)
+    got = s2t.from_interleaved(np.fromfile(state, np.float32), 64)
+    with s2t.Solver(64) as s:
+        s.configure(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5,
+                    dt_max=0.02)
+        s.seed_spectrum(k0=6.0, amplitude=5.0, seed=11)
+        s.step(7)
+        want = s.read_state()
+    assert np.array_equal(got, want)
+    assert d["energy"] > 0

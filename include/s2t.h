@@ -295,6 +295,10 @@ int s2t_profile(s2t_solver* h, int enable);
  * launch adds two event records to the stream (a few microseconds), so a throughput measurement
  * times only the kinds it reports. */
 int s2t_profile_kinds(s2t_solver* h, unsigned mask);
+/* As s2t_profile_kinds, timing the launches of one s2t_step step in `every` (the first, then
+ * every `every`-th step after it), so that a throughput measurement carries fewer event records
+ * while every kernel stage is still sampled. */
+int s2t_profile_sampled(s2t_solver* h, unsigned mask, int every);
 /* Launch count and summed device time (ms) of one kernel kind since profiling was enabled.
  * Waits for queued work. */
 int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);

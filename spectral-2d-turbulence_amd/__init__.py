@@ -37,7 +37,7 @@ EXPORTS = [
     "s2t_read_state", "s2t_reset_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
-    "s2t_profile_kinds", "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
+    "s2t_profile_kinds", "s2t_profile_sampled", "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
     "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
     "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
     "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
@@ -139,6 +139,7 @@ def load(path=None):
         "s2t_rhs": (ip, [vp, u32, dp, fp]),
         "s2t_profile": (ip, [vp, ip]),
         "s2t_profile_kinds": (ip, [vp, ctypes.c_uint]),
+        "s2t_profile_sampled": (ip, [vp, ctypes.c_uint, ctypes.c_int]),
         "s2t_profile_read": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp)]),
         "s2t_kernel_bytes": (ip, [vp, ip, ip, ctypes.POINTER(dp)]),
         "s2t_copy_probe": (ip, [dp, ip, ctypes.POINTER(dp)]),
@@ -350,13 +351,14 @@ class Solver:
         return from_interleaved(out, self.n)
 
     # profiling
-    def profile(self, enable, kinds=None):
-        """Event timing of s2t_step's launches: all kernel kinds, or only `kinds` (PROF_*)."""
-        if enable and kinds is not None:
-            mask = 0
-            for k in kinds:
+    def profile(self, enable, kinds=None, every=1):
+        """Event timing of s2t_step's launches: all kernel kinds, or only `kinds` (PROF_*), in
+        one step of `every`."""
+        if enable and (kinds is not None or every != 1):
+            mask = (1 << 4) - 1 if kinds is None else 0
+            for k in kinds or ():
                 mask |= 1 << int(k)
-            _check(self.lib, self.lib.s2t_profile_kinds(self.h, mask))
+            _check(self.lib, self.lib.s2t_profile_sampled(self.h, mask, int(every)))
         else:
             _check(self.lib, self.lib.s2t_profile(self.h, 1 if enable else 0))
 

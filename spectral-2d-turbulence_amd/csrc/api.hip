@@ -352,9 +352,16 @@ int s2t_profile(s2t_solver* h, int enable) {
 }
 
 int s2t_profile_kinds(s2t_solver* h, unsigned mask) {
+  return s2t_profile_sampled(h, mask, 1);
+}
+
+int s2t_profile_sampled(s2t_solver* h, unsigned mask, int every) {
   SOLVER(h);
+  if (every < 1) return api_bad("profile sampling period must be >= 1");
   S2T_API_BEGIN
   mask &= (1u << S2T_PROF_KINDS) - 1;
+  s->prof_every = every;
+  s->prof_step = 0;
   if (mask) {
     sync(s);
     s->nev = 0;

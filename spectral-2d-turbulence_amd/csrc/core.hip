@@ -28,18 +28,20 @@ static T* dalloc(size_t elems, const char* what) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// profiling: HIP events recorded on the solver's stream around each launch
+// profiling: HIP events recorded on the solver's stream around each launch. They only time:
+// without the system-scope fence a record does not write back and invalidate the caches, which
+// would cost stream time and leave the next kernel's L2 cold
 // ---------------------------------------------------------------------------------------------
 void prof_begin(Solver* s, int kind) {
-  s->prof_open = (s->prof_mask >> kind) & 1u;
+  s->prof_open = ((s->prof_mask >> kind) & 1u) && !s->prof_skip;
   if (!s->prof_open) return;
   if (s->nev == s->cap_ev) {
     const int cap = s->cap_ev ? s->cap_ev * 2 : 256;
     Solver::Ev* ev = (Solver::Ev*)realloc(s->evs, sizeof(Solver::Ev) * cap);
     if (!ev) throw Fail(S2T_ENOMEM, "profiling event list");
     for (int i = s->cap_ev; i < cap; ++i) {
-      hipchk(hipEventCreate(&ev[i].a), "hipEventCreate");
-      hipchk(hipEventCreate(&ev[i].b), "hipEventCreate");
+      hipchk(hipEventCreateWithFlags(&ev[i].a, hipEventDisableSystemFence), "hipEventCreate");
+      hipchk(hipEventCreateWithFlags(&ev[i].b, hipEventDisableSystemFence), "hipEventCreate");
     }
     s->evs = ev;
     s->cap_ev = cap;
@@ -346,6 +348,8 @@ void ensure_primed(Solver* s) {
 
 // encodeStep (+ encodeReduce, + encodeDye) of solver.js:488-545 for all active slots
 void step_once(Solver* s, bool dye) {
+  s->prof_skip = s->prof_every > 1 && s->prof_step % s->prof_every != 0;
+  s->prof_step += 1;
   const bool force = s->splats.count > 0;
   if (force) need_force(s);
   if (dye) need_vel(s);
@@ -364,6 +368,7 @@ void step_once(Solver* s, bool dye) {
     prof_end(s);
     s->dye_parity = 1 - s->dye_parity;
   }
+  s->prof_skip = false;
 }
 
 

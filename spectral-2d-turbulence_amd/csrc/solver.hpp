@@ -70,6 +70,9 @@ struct Solver {
   // profiling
   unsigned prof_mask = 0;  // kernel kinds timed with events (bit k = S2T_PROF kind k)
   bool prof_open = false;  // prof_begin recorded an event that prof_end closes
+  int prof_every = 1;      // time the launches of one step in prof_every
+  int prof_step = 0;       // steps since profiling was enabled
+  bool prof_skip = false;  // the current step is not timed
   struct Ev { hipEvent_t a, b; int kind; };
   Ev* evs = nullptr;
   int nev = 0, cap_ev = 0;

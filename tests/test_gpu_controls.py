@@ -47,6 +47,18 @@ def test_profile_kinds(s2t):
         s.step(1)
         s.profile(False)
         assert s.profile_read(s2t.PROF_ROWS)[0] == 3 and s.profile_read(s2t.PROF_COLS)[0] == 3
+        # one step in four: steps 0 and 4 of 7, all three stages of each
+        s.profile(True, kinds=[s2t.PROF_COLS], every=4)
+        s.step(3)
+        s.step(4)
+        s.profile(False)
+        assert s.profile_read(s2t.PROF_COLS)[0] == 6 and s.profile_read(s2t.PROF_ROWS)[0] == 0
+        with pytest.raises(s2t.S2TError, match="period"):
+            s.profile(True, every=0)
+        s.profile(True)
+        s.step(2)
+        s.profile(False)
+        assert s.profile_read(s2t.PROF_COLS)[0] == 6
 
 
 @pytest.mark.parametrize("n", [128, 1024])

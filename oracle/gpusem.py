@@ -360,9 +360,13 @@ def dye_step(dye, vel, dt, fade=1.0, splats=()):
     corr: s = trace(p, +h); clamp(1.5 phi0(s) - 0.5 phihat(s), min/max of the 4 phi0 texels
           around s) * fade + sum splat colour * dyeAmp * exp(-r^2/sigma^2); alpha = 1.
 
-    Parity unpinned against the reference (no FP64 dye exists there); pinned by invariants:
-    constant fields stay constant, zero velocity is the identity, an integer-texel uniform shift
-    is exact, and the output respects the limiter bounds.
+    The reference's tests hold no dye vectors, so this is pinned by closed forms and invariants:
+    for a uniform half-texel shift every Fourier mode of the field is multiplied by
+    A (3/2 - |A|^2 / 2) fade, A = (1 + exp(-i theta)) / 2 (the BFECC symbol of bilinear filtering:
+    pins the blend, both trace directions, h and the fade); a splat's mass is the Gaussian
+    integral dyeAmp pi sigma^2 times its colour; constant fields stay constant, zero velocity is
+    the identity, an integer-texel uniform shift is exact, and the output respects the limiter
+    bounds.
     """
     m = dye.shape[0]
     phi0 = dye.astype(np.float64)

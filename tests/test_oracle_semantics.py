@@ -162,6 +162,67 @@ def test_dye_invariants():
     assert out[..., :3].max() <= d[..., :3].astype(float).max() + 1e-3
 
 
+@pytest.mark.parametrize("kw", [8, 12])
+def test_dye_bfecc_half_texel_closed_form(kw):
+    """A uniform shift by half a texel of a cosine dye field: bilinear filtering multiplies the
+    mode exp(i theta j) by A = 1 - a + a exp(-i theta) (forward trace, a = 1/2) and the back trace
+    by conj(A), so BFECC gives A (3/2 - |A|^2 / 2) times the mode, then the fade factor.
+    Compared where the limiter cannot bind (the closed-form value inside the range of the two
+    texels around the trace foot); pins the 3/2, -1/2 blend, both trace directions, h = dt / 2 pi
+    and the fade, which the integer shift (A = a pure shift) cannot tell apart."""
+    m, n, dt, fade, amp = 64, 16, 0.05, 0.9, 0.25
+    theta = 2 * np.pi * kw / m
+    j = np.arange(m)
+    phi0 = np.empty((m, m, 4), np.float16)
+    for ch, psi in enumerate((0.0, 0.7, 1.9)):
+        phi0[..., ch] = (0.5 + amp * np.cos(theta * j + psi))[None, :]
+    phi0[..., 3] = 1
+    vel = np.zeros((n, n, 2))
+    vel[..., 0] = 0.5 * TWO_PI / (dt * m)      # h u m = 1/2 texel
+    out = gs.dye_step(phi0, vel, dt, fade=fade).astype(np.float64)
+    x = phi0.astype(np.float64)
+    th = 2 * np.pi * np.fft.fftfreq(m, 1.0 / m) / m
+    Ak = 0.5 + 0.5 * np.exp(-1j * th)
+    for ch in range(3):
+        # the stored field is the fp16 rounding of the cosine: apply the symbol to all its modes
+        spec = np.fft.fft(x[0, :, ch])
+        want = np.real(np.fft.ifft(spec * Ak * (1.5 - 0.5 * abs(Ak) ** 2))) * fade
+        lo = np.minimum(x[0, :, ch], np.roll(x[0, :, ch], 1)) * fade
+        hi = np.maximum(x[0, :, ch], np.roll(x[0, :, ch], 1)) * fade
+        free = (want > lo + 2e-3) & (want < hi - 2e-3)
+        assert free.sum() > m // 2
+        got = out[:, :, ch]
+        # fp16 intermediates: phi1 and phihat are stored at half precision, the output too
+        assert np.max(np.abs(got[:, free] - want[None, free])) < 1.5e-3
+        # the blend is visible: plain semi-Lagrangian (gain A alone) misses by far more
+        plain = np.real(np.fft.ifft(spec * Ak)) * fade
+        assert np.max(np.abs(plain[free] - want[free])) > 5e-3
+    assert np.all(out[..., 3] == 1)
+
+
+@pytest.mark.parametrize("cx, cy", [(0.3, 0.6), (0.98, 0.01)])
+def test_dye_splat_gaussian_integral(cx, cy):
+    """A dye splat on a black field at rest: its mass is the Gaussian integral dyeAmp pi sigma^2
+    times the colour (cut off at r = 3 sigma, a factor 1 - e^-9 that this bound does not resolve),
+    it peaks at the texel nearest the centre with dyeAmp exp(-r^2 / sigma^2) there, and it wraps
+    periodically (a splat at a corner keeps its mass). Pins the width (sigma^2, not 2 sigma^2),
+    the amplitude, the colour and the periodic separation."""
+    m, n, sig, amp = 256, 16, 0.04, 0.8
+    colour = (1.0, 0.5, 0.25)
+    d = np.zeros((m, m, 4), np.float16)
+    d[..., 3] = 1
+    sp = dict(x=cx, y=cy, radius=sig, dyeAmp=amp, color=colour)
+    out = gs.dye_step(d, np.zeros((n, n, 2)), 0.01, fade=1.0, splats=[sp]).astype(np.float64)
+    mass = amp * np.pi * sig ** 2 * (1 - np.exp(-9.0))
+    for ch, c in enumerate(colour):
+        assert abs(out[..., ch].sum() / m ** 2 / (c * mass) - 1) < 2e-3
+    iy, ix = np.unravel_index(np.argmax(out[..., 0]), (m, m))
+    r2 = (((ix + 0.5) / m - cx + 0.5) % 1 - 0.5) ** 2 + (((iy + 0.5) / m - cy + 0.5) % 1 - 0.5) ** 2
+    assert r2 <= 0.5 / m ** 2
+    assert abs(out[iy, ix, 0] - amp * np.exp(-r2 / sig ** 2)) < 1e-3
+    assert np.all(out[..., 3] == 1)
+
+
 # ---------------------------------------------------------------------------------------------
 # band-limited random seed (s2t_bandlimited_spectrum)
 # ---------------------------------------------------------------------------------------------

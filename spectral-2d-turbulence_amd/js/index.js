@@ -119,6 +119,12 @@ class Solver {
     addon().step(this._h(), count, !!dye);
   }
 
+  // the same steps on a worker thread: the Promise resolves once the device has finished them
+  // (calls on the solver meanwhile wait for them)
+  stepAsync(count = 1, { dye = false } = {}) {
+    return addon().stepAsync(this._h(), count, !!dye);
+  }
+
   encodeStep() {
     this.step(1);
   }

@@ -1,11 +1,12 @@
 // N-API addon over libs2t.so (include/s2t.h). Thin: every function converts JS values, calls one
 // C entry point and throws a JS Error carrying s2t_last_error() on a negative status (the
 // reference throws Error for programming errors, src/gpu/fft.js:28,84). Long device work that
-// returns data (diagnostics) is also offered as a Promise on a worker thread, like the reference's
-// async readDiagnostics (src/gpu/solver.js:570-608). The serving loop reports each request's final
+// returns data (diagnostics), and a run of steps, are also offered as Promises on a worker thread,
+// like the reference's async readDiagnostics (src/gpu/solver.js:570-608). The serving loop reports each request's final
 // status from its native worker thread through a thread-safe function.
 #include <node_api.h>
 
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -52,13 +53,64 @@ bool get_args(napi_env env, napi_callback_info info, Args& a, size_t need) {
   return true;
 }
 
-s2t_solver* handle(napi_env env, napi_value v) {
+// A solver handle. Calls on one solver must not overlap (the C library's contract), and the
+// Promise-returning calls run on libuv worker threads. Every call takes a ticket on the JS thread
+// when it is made and runs when its turn comes, so calls take effect in the order the program
+// made them (as work encoded into the reference's GPU queue does): a synchronous call made while
+// a Promise-returning one is pending waits for it.
+struct SolverBox {
+  s2t_solver* h = nullptr;
+  std::mutex mu;
+  std::condition_variable cv;
+  uint64_t issued = 0, served = 0;
+  uint64_t ticket() {
+    std::lock_guard<std::mutex> lk(mu);
+    return issued++;
+  }
+  void begin(uint64_t t) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return served == t; });
+  }
+  void end() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      served += 1;
+    }
+    cv.notify_all();
+  }
+};
+
+// the solver of a handle for the rest of a call, in its turn (empty on a bad handle, with a JS
+// TypeError pending)
+class Held {
+ public:
+  Held() = default;
+  explicit Held(SolverBox* b) : b_(b) { b_->begin(b_->ticket()); }
+  Held(Held&& o) noexcept : b_(o.b_) { o.b_ = nullptr; }
+  Held(const Held&) = delete;
+  Held& operator=(const Held&) = delete;
+  ~Held() {
+    if (b_) b_->end();
+  }
+  explicit operator bool() const { return b_ != nullptr; }
+  operator s2t_solver*() const { return b_ ? b_->h : nullptr; }
+
+ private:
+  SolverBox* b_ = nullptr;
+};
+
+SolverBox* solver_box(napi_env env, napi_value v) {
   void* p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_type_error(env, "S2T_HANDLE", "not a solver handle (destroyed?)");
     return nullptr;
   }
-  return static_cast<s2t_solver*>(p);
+  return static_cast<SolverBox*>(p);
+}
+
+Held handle(napi_env env, napi_value v) {
+  SolverBox* b = solver_box(env, v);
+  return b ? Held(b) : Held();
 }
 
 double num(napi_env env, napi_value v, double dflt) {
@@ -113,7 +165,12 @@ napi_value make_num(napi_env env, double d) {
   return v;
 }
 
-void finalize_solver(napi_env, void* data, void*) { s2t_destroy(static_cast<s2t_solver*>(data)); }
+// runs when no JS value and no pending job refers to the handle any more
+void finalize_solver(napi_env, void* data, void*) {
+  SolverBox* b = static_cast<SolverBox*>(data);
+  s2t_destroy(b->h);
+  delete b;
+}
 
 // ---------------------------------------------------------------------------------------------
 
@@ -123,8 +180,14 @@ napi_value Create(napi_env env, napi_callback_info info) {
   const int n = (int)num(env, a.argv[0], 256), dye = (int)num(env, a.argv[1], 2);
   s2t_solver* h = nullptr;
   S2T_CALL(env, s2t_create(n, dye, &h));
+  SolverBox* b = new SolverBox();
+  b->h = h;
   napi_value ext;
-  NAPI_CALL(env, napi_create_external(env, h, finalize_solver, nullptr, &ext));
+  if (napi_create_external(env, b, finalize_solver, nullptr, &ext) != napi_ok) {
+    finalize_solver(env, b, nullptr);
+    napi_throw_error(env, "S2T_NAPI", "napi_create_external failed");
+    return nullptr;
+  }
   return ext;
 }
 
@@ -163,7 +226,7 @@ s2t_params params_of(napi_env env, napi_value c) {
 napi_value Configure(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const s2t_params p = params_of(env, a.argv[1]);
   S2T_CALL(env, s2t_configure(h, &p));
@@ -173,7 +236,7 @@ napi_value Configure(napi_env env, napi_callback_info info) {
 napi_value NuH(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   double v = 0;
   S2T_CALL(env, s2t_nu_h(h, &v));
@@ -184,7 +247,7 @@ napi_value NuH(napi_env env, napi_callback_info info) {
 napi_value SetSplats(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   uint32_t len = 0;
   NAPI_CALL(env, napi_get_array_length(env, a.argv[1], &len));
@@ -218,7 +281,7 @@ napi_value SetSplats(napi_env env, napi_callback_info info) {
 napi_value SeedSpectrum(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 4)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const double seed = num(env, a.argv[3], 1);
   S2T_CALL(env, s2t_seed_spectrum(h, num(env, a.argv[1], 14), num(env, a.argv[2], 6),
@@ -229,7 +292,7 @@ napi_value SeedSpectrum(napi_env env, napi_callback_info info) {
 napi_value SeedPhysical(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const int n = (int)num(env, a.argv[2], 0);
   size_t cnt = 0;
@@ -246,7 +309,7 @@ napi_value SeedPhysical(napi_env env, napi_callback_info info) {
 napi_value WriteState(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const int n = (int)num(env, a.argv[2], 0);
   size_t cnt = 0;
@@ -263,7 +326,7 @@ napi_value WriteState(napi_env env, napi_callback_info info) {
 napi_value ReadState(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const int n = (int)num(env, a.argv[1], 0);
   void* data = nullptr;
@@ -276,7 +339,7 @@ napi_value ReadState(napi_env env, napi_callback_info info) {
 napi_value ResetClock(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   S2T_CALL(env, s2t_reset_clock(h));
   return nullptr;
@@ -285,7 +348,7 @@ napi_value ResetClock(napi_env env, napi_callback_info info) {
 napi_value Step(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   bool dye = false;
   napi_get_value_bool(env, a.argv[2], &dye);
@@ -296,105 +359,169 @@ napi_value Step(napi_env env, napi_callback_info info) {
 napi_value Sync(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   S2T_CALL(env, s2t_sync(h));
   return nullptr;
 }
 
-// diagnostics record (solver.js:591-602) in FP64
-struct DiagJob {
-  s2t_solver* h = nullptr;
-  int n = 0;
+// Work that runs on a libuv worker thread and settles a Promise: it holds a reference to the
+// solver's handle (so the solver outlives it) and runs in the turn of the call that queued it.
+struct Job {
+  SolverBox* box = nullptr;
+  napi_ref ref = nullptr;
+  uint64_t turn = 0;
   int rc = 0;
   std::string err;
-  s2t_diag d{};
-  std::vector<double> spectrum, zonal;
   napi_deferred deferred = nullptr;
   napi_async_work work = nullptr;
+  virtual ~Job() = default;
+  virtual int run(s2t_solver* h) = 0;
+  virtual napi_value result(napi_env env) = 0;
+  void execute() {
+    box->begin(turn);
+    rc = run(box->h);
+    if (rc != S2T_OK) err = s2t_last_error();  // thread-local: read on the same thread
+    box->end();
+  }
 };
 
-napi_value diag_object(napi_env env, DiagJob& j) {
+void job_execute(napi_env, void* data) { static_cast<Job*>(data)->execute(); }
+
+void job_complete(napi_env env, napi_status, void* data) {
+  Job* j = static_cast<Job*>(data);
+  if (j->rc == S2T_OK) {
+    napi_resolve_deferred(env, j->deferred, j->result(env));
+  } else {
+    napi_value msg, code, err;
+    const std::string c = "S2T_" + std::to_string(j->rc);
+    napi_create_string_utf8(env, c.c_str(), NAPI_AUTO_LENGTH, &code);
+    napi_create_string_utf8(env, j->err.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, code, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_reference(env, j->ref);
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+
+// queues j for the solver handle v; returns its Promise (j is owned by the queue from here on)
+napi_value queue_job(napi_env env, napi_value v, Job* j, const char* what) {
+  j->box = solver_box(env, v);
+  napi_value promise = nullptr, name;
+  if (!j->box || napi_create_reference(env, v, 1, &j->ref) != napi_ok) {
+    delete j;
+    return nullptr;
+  }
+  bool ok = napi_create_promise(env, &j->deferred, &promise) == napi_ok &&
+            napi_create_string_utf8(env, what, NAPI_AUTO_LENGTH, &name) == napi_ok &&
+            napi_create_async_work(env, nullptr, name, job_execute, job_complete, j, &j->work) ==
+                napi_ok;
+  if (ok) {
+    j->turn = j->box->ticket();
+    ok = napi_queue_async_work(env, j->work) == napi_ok;
+    if (!ok) {
+      // give the turn back in order
+      j->box->begin(j->turn);
+      j->box->end();
+    }
+  }
+  if (!ok) {
+    napi_delete_reference(env, j->ref);
+    if (j->work) napi_delete_async_work(env, j->work);
+    delete j;
+    napi_throw_error(env, "S2T_NAPI", "could not queue the work");
+    return nullptr;
+  }
+  return promise;
+}
+
+// diagnostics record (solver.js:591-602) in FP64
+struct DiagJob : Job {
+  int n = 0;
+  s2t_diag d{};
+  std::vector<double> spectrum, zonal;
+  int run(s2t_solver* h) override {
+    spectrum.assign(n / 2 + 1, 0.0);
+    zonal.assign(n, 0.0);
+    return s2t_diagnostics(h, &d, spectrum.data(), zonal.data());
+  }
+  napi_value result(napi_env env) override;
+};
+
+napi_value DiagJob::result(napi_env env) {
   napi_value obj;
   napi_create_object(env, &obj);
   void* data = nullptr;
-  napi_value spec = new_typed(env, napi_float64_array, j.spectrum.size(), 8, &data);
-  memcpy(data, j.spectrum.data(), j.spectrum.size() * 8);
-  napi_value zon = new_typed(env, napi_float64_array, j.zonal.size(), 8, &data);
-  memcpy(data, j.zonal.data(), j.zonal.size() * 8);
+  napi_value spec = new_typed(env, napi_float64_array, spectrum.size(), 8, &data);
+  memcpy(data, spectrum.data(), spectrum.size() * 8);
+  napi_value zon = new_typed(env, napi_float64_array, zonal.size(), 8, &data);
+  memcpy(data, zonal.data(), zonal.size() * 8);
   napi_set_named_property(env, obj, "spectrum", spec);
   napi_set_named_property(env, obj, "zonal", zon);
-  napi_set_named_property(env, obj, "energy", make_num(env, j.d.energy));
-  napi_set_named_property(env, obj, "enstrophy", make_num(env, j.d.enstrophy));
-  napi_set_named_property(env, obj, "dt", make_num(env, j.d.dt));
-  napi_set_named_property(env, obj, "maxSpeed", make_num(env, j.d.max_speed));
-  napi_set_named_property(env, obj, "time", make_num(env, j.d.time));
-  napi_set_named_property(env, obj, "stepIndex", make_num(env, j.d.step_index));
+  napi_set_named_property(env, obj, "energy", make_num(env, d.energy));
+  napi_set_named_property(env, obj, "enstrophy", make_num(env, d.enstrophy));
+  napi_set_named_property(env, obj, "dt", make_num(env, d.dt));
+  napi_set_named_property(env, obj, "maxSpeed", make_num(env, d.max_speed));
+  napi_set_named_property(env, obj, "time", make_num(env, d.time));
+  napi_set_named_property(env, obj, "stepIndex", make_num(env, d.step_index));
   napi_value nf;
-  napi_get_boolean(env, j.d.nonfinite != 0, &nf);
+  napi_get_boolean(env, d.nonfinite != 0, &nf);
   napi_set_named_property(env, obj, "nonfinite", nf);
   return obj;
-}
-
-void run_diag(DiagJob& j) {
-  j.spectrum.assign(j.n / 2 + 1, 0.0);
-  j.zonal.assign(j.n, 0.0);
-  j.rc = s2t_diagnostics(j.h, &j.d, j.spectrum.data(), j.zonal.data());
-  if (j.rc != S2T_OK) j.err = s2t_last_error();
 }
 
 napi_value Diagnostics(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
   DiagJob j;
-  j.h = handle(env, a.argv[0]);
-  if (!j.h) return nullptr;
   j.n = (int)num(env, a.argv[1], 0);
-  run_diag(j);
-  if (j.rc != S2T_OK) return throw_status(env, j.rc);
-  return diag_object(env, j);
+  S2T_CALL(env, j.run(h));
+  return j.result(env);
 }
 
-void diag_execute(napi_env, void* data) { run_diag(*static_cast<DiagJob*>(data)); }
-
-void diag_complete(napi_env env, napi_status, void* data) {
-  DiagJob* j = static_cast<DiagJob*>(data);
-  if (j->rc == S2T_OK) {
-    napi_resolve_deferred(env, j->deferred, diag_object(env, *j));
-  } else {
-    napi_value msg, err;
-    napi_create_string_utf8(env, j->err.c_str(), NAPI_AUTO_LENGTH, &msg);
-    napi_create_error(env, nullptr, msg, &err);
-    napi_reject_deferred(env, j->deferred, err);
-  }
-  napi_delete_async_work(env, j->work);
-  delete j;
-}
-
-// diagnosticsAsync(h, n, keepAlive) -> Promise; keepAlive is held by the caller for the duration
+// diagnosticsAsync(h, n) -> Promise of the diagnostics record
 napi_value DiagnosticsAsync(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
   DiagJob* j = new DiagJob();
-  j->h = handle(env, a.argv[0]);
-  if (!j->h) {
-    delete j;
-    return nullptr;
-  }
   j->n = (int)num(env, a.argv[1], 0);
-  napi_value promise, name;
-  NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
-  NAPI_CALL(env, napi_create_string_utf8(env, "s2t_diagnostics", NAPI_AUTO_LENGTH, &name));
-  NAPI_CALL(env, napi_create_async_work(env, nullptr, name, diag_execute, diag_complete, j,
-                                        &j->work));
-  NAPI_CALL(env, napi_queue_async_work(env, j->work));
-  return promise;
+  return queue_job(env, a.argv[0], j, "s2t_diagnostics");
+}
+
+// `count` steps (+ dye), then a wait for the device: the Promise settles when they are done
+struct StepJob : Job {
+  int count = 1;
+  uint32_t flags = 0;
+  int run(s2t_solver* h) override {
+    const int rc = s2t_step(h, count, flags);
+    return rc != S2T_OK ? rc : s2t_sync(h);
+  }
+  napi_value result(napi_env env) override {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    return u;
+  }
+};
+
+// stepAsync(h, count, dye) -> Promise
+napi_value StepAsync(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  StepJob* j = new StepJob();
+  j->count = (int)num(env, a.argv[1], 1);
+  bool dye = false;
+  napi_get_value_bool(env, a.argv[2], &dye);
+  j->flags = dye ? S2T_STEP_DYE : 0u;
+  return queue_job(env, a.argv[0], j, "s2t_step");
 }
 
 napi_value PaintDye(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   S2T_CALL(env, s2t_paint_dye(h, (int)num(env, a.argv[1], 1)));
   return nullptr;
@@ -403,7 +530,7 @@ napi_value PaintDye(napi_env env, napi_callback_info info) {
 napi_value ReadDye(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   int m = 0;
   S2T_CALL(env, s2t_dye_size(h, &m));
@@ -418,7 +545,7 @@ napi_value ReadDye(napi_env env, napi_callback_info info) {
 napi_value Fft2(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 4)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   size_t cnt = 0;
   float* in = typed<float>(env, a.argv[1], napi_float32_array, &cnt);
@@ -440,7 +567,7 @@ napi_value Fft2(napi_env env, napi_callback_info info) {
 napi_value VelocityPack(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   size_t cnt = 0;
   float* w = typed<float>(env, a.argv[1], napi_float32_array, &cnt);
@@ -466,7 +593,7 @@ napi_value VelocityPack(napi_env env, napi_callback_info info) {
 napi_value Assemble(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 6)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   size_t cnt = 0, cf = 0;
   float* z = typed<float>(env, a.argv[1], napi_float32_array, &cnt);
@@ -494,7 +621,7 @@ napi_value Assemble(napi_env env, napi_callback_info info) {
 napi_value Rhs(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 4)) return nullptr;
-  s2t_solver* h = handle(env, a.argv[0]);
+  Held h = handle(env, a.argv[0]);
   if (!h) return nullptr;
   const int n = (int)num(env, a.argv[1], 0);
   void* data = nullptr;
@@ -797,6 +924,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readState", ReadState},   {"resetClock", ResetClock},
              {"step", Step},             {"sync", Sync},
              {"diagnostics", Diagnostics}, {"diagnosticsAsync", DiagnosticsAsync},
+             {"stepAsync", StepAsync},
              {"paintDye", PaintDye},     {"readDye", ReadDye},
              {"fft2", Fft2},             {"velocityPack", VelocityPack},
              {"assemble", Assemble},     {"rhs", Rhs},

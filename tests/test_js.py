@@ -123,3 +123,41 @@ def test_js_server_promises(s2t, tmp_path):
         want = s.read_state()
     assert np.array_equal(got, want)
     assert d["energy"] > 0
+
+
+ORDER_JS = r"""
+const m = require('./index.js');
+const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };
+(async () => {
+  const a = await m.Solver.create(null, { n: 128, dyeScale: 1 });
+  const b = await m.Solver.create(null, { n: 128, dyeScale: 1 });
+  for (const s of [a, b]) { s.configure(phys); s.seed({ k0: 8, amplitude: 5, seed: 3 }); }
+  // calls take effect in program order whichever thread runs them
+  const d0 = a.readDiagnostics();
+  a.step(2);
+  const st = a.stepAsync(3);
+  const d5 = a.readDiagnostics();
+  a.step(1);
+  const [r0, r5] = await Promise.all([d0, d5]);
+  await st;
+  b.step(6);
+  const sa = a.readState(), sb = b.readState();
+  let same = sa.length === sb.length;
+  for (let i = 0; same && i < sa.length; i++) same = sa[i] === sb[i] || (sa[i] !== sa[i] && sb[i] !== sb[i]);
+  console.log(JSON.stringify({ i0: r0.stepIndex, i5: r5.stepIndex, i6: a.diagnostics().stepIndex,
+                               same, e0: r0.energy, e5: r5.energy }));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_async_calls_keep_program_order():
+    """stepAsync and readDiagnostics run on worker threads, but every call on a solver takes
+    effect in the order the program made it (a diagnostics read issued before some steps sees
+    the state before them), and stepAsync steps are the synchronous steps, bit for bit."""
+    import json
+    r = subprocess.run([NODE, "-e", ORDER_JS], cwd=JS, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert (out["i0"], out["i5"], out["i6"]) == (0, 5, 6)
+    assert out["same"] and out["e0"] > out["e5"] > 0

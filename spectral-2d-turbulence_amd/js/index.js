@@ -309,6 +309,61 @@ const SCENARIOS = {
 
 const SCENARIO_NAMES = Object.keys(SCENARIOS);
 
+// Ensemble (include/s2t.h s2t_batch_*): members of mixed grid sizes step together, members of
+// one size in the same fused kernels, each bit-identical to the same member run as a Solver.
+class Batch {
+  constructor(sizes) {
+    this.sizes = sizes.slice();
+    this.handle = addon().batchCreate(this.sizes);
+    this.cfg = this.sizes.map(() => ({ ...DEFAULT_CONFIG }));
+    this.sizes.forEach((_, i) => this.configure(i, {}));
+  }
+
+  get count() {
+    return this.sizes.length;
+  }
+
+  configure(i, cfg) {
+    this.cfg[i] = { ...this.cfg[i], ...cfg };
+    addon().batchConfigure(this.handle, i, this.cfg[i]);
+  }
+
+  seed(i, { k0 = 14, amplitude = 6, seed = 1 } = {}) {
+    addon().batchSeedSpectrum(this.handle, i, k0, amplitude, seed);
+  }
+
+  seedPhysical(i, values) {
+    const f = values instanceof Float32Array ? values : Float32Array.from(values);
+    addon().batchSeedPhysical(this.handle, i, f);
+  }
+
+  // one hipGraph per RK3 step of all members (default) or eager launches
+  useGraphs(on) {
+    addon().batchUseGraphs(this.handle, !!on);
+  }
+
+  step(count = 1) {
+    addon().batchStep(this.handle, count);
+  }
+
+  sync() {
+    addon().batchSync(this.handle);
+  }
+
+  readState(i) {
+    return addon().batchReadState(this.handle, i);
+  }
+
+  writeState(i, spectrum) {
+    const f = spectrum instanceof Float32Array ? spectrum : Float32Array.from(spectrum);
+    addon().batchWriteState(this.handle, i, f);
+  }
+
+  diagnostics(i) {
+    return addon().batchDiagnostics(this.handle, i);
+  }
+}
+
 // request statuses and server-side scenarios of include/s2t.h
 const REQ = Object.freeze({ QUEUED: 0, RUNNING: 1, DONE: 2, REJECTED: 3, CANCELLED: 4 });
 const SERVER_SCENARIOS = Object.freeze({ turbulence: 0, 'taylor-green': 1, dipole: 2, merger: 3 });
@@ -395,6 +450,7 @@ class Server {
 
 module.exports = {
   Solver,
+  Batch,
   Server,
   REQ,
   Fft2D,

@@ -633,6 +633,159 @@ napi_value Rhs(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// ensembles (s2t_batch_*)
+// ---------------------------------------------------------------------------------------------
+void finalize_batch(napi_env, void* data, void*) { s2t_batch_destroy(static_cast<s2t_batch*>(data)); }
+
+s2t_batch* batch(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, "S2T_HANDLE", "not a batch handle");
+    return nullptr;
+  }
+  return static_cast<s2t_batch*>(p);
+}
+
+bool int_array(napi_env env, napi_value v, std::vector<int>& out);
+
+// batchCreate(sizes[]) -> handle
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  std::vector<int> sizes;
+  if (!int_array(env, a.argv[0], sizes)) return nullptr;
+  s2t_batch* b = nullptr;
+  S2T_CALL(env, s2t_batch_create(sizes.data(), (int)sizes.size(), &b));
+  napi_value ext;
+  if (napi_create_external(env, b, finalize_batch, nullptr, &ext) != napi_ok) {
+    s2t_batch_destroy(b);
+    napi_throw_error(env, "S2T_NAPI", "napi_create_external failed");
+    return nullptr;
+  }
+  return ext;
+}
+
+// the member's grid size, or 0 with a JS error pending
+int member_size(napi_env env, s2t_batch* b, int i) {
+  int n = 0;
+  if (s2t_batch_size(b, i, &n) != S2T_OK) {
+    throw_status(env, S2T_EINVAL);
+    return 0;
+  }
+  return n;
+}
+
+#define BATCH_MEMBER(argc)                          \
+  Args a;                                           \
+  if (!get_args(env, info, a, (argc))) return nullptr; \
+  s2t_batch* b = batch(env, a.argv[0]);             \
+  if (!b) return nullptr;                           \
+  const int i = (int)num(env, a.argv[1], -1)
+
+napi_value BatchSize(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(2);
+  const int n = member_size(env, b, i);
+  return n ? make_num(env, n) : nullptr;
+}
+
+napi_value BatchConfigure(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(3);
+  const s2t_params p = params_of(env, a.argv[2]);
+  S2T_CALL(env, s2t_batch_configure(b, i, &p));
+  return nullptr;
+}
+
+napi_value BatchSeedSpectrum(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(5);
+  S2T_CALL(env, s2t_batch_seed_spectrum(b, i, num(env, a.argv[2], 14), num(env, a.argv[3], 6),
+                                        (uint32_t)(int64_t)num(env, a.argv[4], 1)));
+  return nullptr;
+}
+
+// batchSeedPhysical(h, i, Float32Array n*n) / batchWriteState(h, i, Float32Array 2*n*n)
+napi_value BatchSeedPhysical(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(3);
+  const int n = member_size(env, b, i);
+  if (!n) return nullptr;
+  size_t cnt = 0;
+  float* f = typed<float>(env, a.argv[2], napi_float32_array, &cnt);
+  if (!f) return nullptr;
+  if (cnt != (size_t)n * n) {
+    napi_throw_range_error(env, "S2T_ARGS", "field must hold n*n values");
+    return nullptr;
+  }
+  S2T_CALL(env, s2t_batch_seed_physical(b, i, f));
+  return nullptr;
+}
+
+napi_value BatchWriteState(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(3);
+  const int n = member_size(env, b, i);
+  if (!n) return nullptr;
+  size_t cnt = 0;
+  float* f = typed<float>(env, a.argv[2], napi_float32_array, &cnt);
+  if (!f) return nullptr;
+  if (cnt != 2 * (size_t)n * n) {
+    napi_throw_range_error(env, "S2T_ARGS", "spectrum must hold 2*n*n values");
+    return nullptr;
+  }
+  S2T_CALL(env, s2t_batch_write_state(b, i, f));
+  return nullptr;
+}
+
+napi_value BatchReadState(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(2);
+  const int n = member_size(env, b, i);
+  if (!n) return nullptr;
+  void* data = nullptr;
+  napi_value arr = new_typed(env, napi_float32_array, 2 * (size_t)n * n, 4, &data);
+  if (!arr) return nullptr;
+  S2T_CALL(env, s2t_batch_read_state(b, i, static_cast<float*>(data)));
+  return arr;
+}
+
+napi_value BatchDiagnostics(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(2);
+  const int n = member_size(env, b, i);
+  if (!n) return nullptr;
+  DiagJob j;
+  j.n = n;
+  j.spectrum.assign(n / 2 + 1, 0.0);
+  j.zonal.assign(n, 0.0);
+  S2T_CALL(env, s2t_batch_diagnostics(b, i, &j.d, j.spectrum.data(), j.zonal.data()));
+  return j.result(env);
+}
+
+napi_value BatchUseGraphs(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  s2t_batch* b = batch(env, a.argv[0]);
+  if (!b) return nullptr;
+  bool on = true;
+  napi_get_value_bool(env, a.argv[1], &on);
+  S2T_CALL(env, s2t_batch_use_graphs(b, on ? 1 : 0));
+  return nullptr;
+}
+
+napi_value BatchStep(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  s2t_batch* b = batch(env, a.argv[0]);
+  if (!b) return nullptr;
+  S2T_CALL(env, s2t_batch_step(b, (int)num(env, a.argv[1], 1)));
+  return nullptr;
+}
+
+napi_value BatchSync(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  s2t_batch* b = batch(env, a.argv[0]);
+  if (!b) return nullptr;
+  S2T_CALL(env, s2t_batch_sync(b));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------------------------
 // serving loop (s2t_server_*)
 // ---------------------------------------------------------------------------------------------
 struct ServerBox {
@@ -931,7 +1084,13 @@ napi_value Init(napi_env env, napi_value exports) {
              {"serverCreate", ServerCreate}, {"serverSubmit", ServerSubmit},
              {"serverPoll", ServerPoll}, {"serverCancel", ServerCancel},
              {"serverReadState", ServerReadState}, {"serverStats", ServerStats},
-             {"serverRef", ServerRef}, {"serverDestroy", ServerDestroy}};
+             {"serverRef", ServerRef}, {"serverDestroy", ServerDestroy},
+             {"batchCreate", BatchCreate}, {"batchSize", BatchSize},
+             {"batchConfigure", BatchConfigure}, {"batchSeedSpectrum", BatchSeedSpectrum},
+             {"batchSeedPhysical", BatchSeedPhysical}, {"batchWriteState", BatchWriteState},
+             {"batchReadState", BatchReadState}, {"batchDiagnostics", BatchDiagnostics},
+             {"batchUseGraphs", BatchUseGraphs}, {"batchStep", BatchStep},
+             {"batchSync", BatchSync}};
   for (const Fn& f : fns) {
     napi_value fn;
     NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

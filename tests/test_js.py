@@ -161,3 +161,47 @@ def test_js_async_calls_keep_program_order():
     out = json.loads(r.stdout)
     assert (out["i0"], out["i5"], out["i6"]) == (0, 5, 6)
     assert out["same"] and out["e0"] > out["e5"] > 0
+
+
+BATCH_JS = r"""
+const m = require('./index.js');
+const cfgs = [
+  { beta: 0, drag: 0.01, viscosity: 1, hyperP: 4 },
+  { beta: 8, drag: 0.02, viscosity: 0.5, hyperP: 1, forceAmp: 2, forceK: 10, forceWidth: 2 },
+  { beta: 3, drag: 0, viscosity: 2, hyperP: 2 },
+];
+const sizes = [64, 128, 64];
+(async () => {
+  const b = new m.Batch(sizes);
+  sizes.forEach((n, i) => { b.configure(i, cfgs[i]); b.seed(i, { k0: 6, amplitude: 5, seed: 10 + i }); });
+  b.step(7);
+  const out = [];
+  for (let i = 0; i < sizes.length; i++) {
+    const s = await m.Solver.create(null, { n: sizes[i], dyeScale: 1 });
+    s.configure(cfgs[i]);
+    s.seed({ k0: 6, amplitude: 5, seed: 10 + i });
+    s.step(7);
+    const x = b.readState(i), y = s.readState();
+    let same = x.length === y.length;
+    for (let k = 0; same && k < x.length; k++) same = x[k] === y[k];
+    out.push({ same, step: b.diagnostics(i).stepIndex, e: b.diagnostics(i).energy });
+  }
+  let threw = false;
+  try { b.readState(3); } catch (e) { threw = true; }
+  console.log(JSON.stringify({ out, threw }));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_batch_members_equal_plain_solvers():
+    """The JS Batch over s2t_batch_*: members of mixed sizes and parameters (forcing included)
+    step together and each equals the same member run as a JS Solver, bit for bit; a member
+    index out of range throws."""
+    import json
+    r = subprocess.run([NODE, "-e", BATCH_JS], cwd=JS, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["threw"]
+    for rec in out["out"]:
+        assert rec["same"] and rec["step"] == 7 and rec["e"] > 0

@@ -156,6 +156,33 @@ def test_one_step_4096_vs_oracle(s2t):
     assert err < 1e-5
 
 
+def test_forced_steps_4096_vs_oracle(s2t):
+    """The bench workload at its own size: forced beta-plane turbulence at 4096^2 (beta 10,
+    hyperviscosity p = 4 at rate 1, stochastic forcing at k_f = 32 +- 2 with the amplitude of the
+    closed-form injection rate 1e-3, adaptive CFL dt), two steps from a seeded state, against the
+    FP64 oracle given the device's dt and step index: relative error <= 1e-5, the bound of one
+    unforced step."""
+    n = 4096
+    amp = gs.amp_for_rate(1e-3, n, 32.0, 2.0)
+    params = dict(beta=10.0, drag=0.01, viscosity=1.0, hyper_p=4.0, force_amp=amp, force_k=32.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    o = oracle_for(params, n)
+    o.w = gs.fill_spectrum(n, k0=64.0, amplitude=2.0, seed=9).astype(np.complex128)
+    dts = []
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.seed_spectrum(k0=64.0, amplitude=2.0, seed=9)
+        for _ in range(2):
+            s.step(1)
+            dts.append(s.diagnostics()["dt"])
+        got = s.read_state()
+    for k, dt in enumerate(dts):
+        o.step(dt=dt, forcing=gs.stochastic_forcing(n, amp, 32.0, 2.0, k, dt))
+    err = rel_err(got, o.w)
+    print("4096 forced two-step relative error vs FP64: %.3e" % err)
+    assert err < 1e-5
+
+
 def test_twenty_steps_cfl_vs_oracle(s2t):
     n = 256
     params = dict(beta=5.0, drag=0.0, viscosity=0.5, hyper_p=4.0, force_amp=0.0, force_k=8.0,

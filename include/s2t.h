@@ -213,6 +213,11 @@ int s2t_batch_use_graphs(s2t_batch* b, int enable);
 /* nsteps steps of every member; enqueues only. */
 int s2t_batch_step(s2t_batch* b, int nsteps);
 int s2t_batch_sync(s2t_batch* b);
+/* s2t_autotune for every size class of the batch, each with its own member count (replaces the
+ * captured steps). */
+int s2t_batch_autotune(s2t_batch* b, int steps);
+/* Column-group width in effect for a member's size class. */
+int s2t_batch_column_width(s2t_batch* b, int member, int* width);
 
 /* ------------------------------------------------------------------------------------------ */
 /* serving: continuous batching of simulation requests (no counterpart in the reference)        */
@@ -281,6 +286,20 @@ int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);
 /* ------------------------------------------------------------------------------------------ */
 /* profiling: per-kernel HIP-event timing on the solver's stream                                */
 /* ------------------------------------------------------------------------------------------ */
+
+/* ------------------------------------------------------------------------------------------ */
+/* Tuning, with no counterpart in the reference (its schedules are fixed per grid size). The step's
+ * column kernel is compiled for a few column-group widths (columns per workgroup) per grid size;
+ * every width gives bit-identical results, and the best one depends on the size and on how many
+ * instances share the launches. */
+/* Compiled widths for grid size n, the default first (host only). */
+int s2t_column_widths(int n, int* widths, int max, int* count);
+/* Use a compiled width for this solver's steps (0: the size's default). */
+int s2t_set_column_width(s2t_solver* h, int width);
+/* Times `steps` steps per compiled width on a scratch solver of the same size and parameters (the
+ * solver's own state is not touched) and keeps the fastest: *width, and the milliseconds per step
+ * of each width in s2t_column_widths order in ms_per_step[0 .. max-1]. */
+int s2t_autotune(s2t_solver* h, int steps, int* width, double* ms_per_step, int max);
 
 #define S2T_PROF_ROWS 0  /* fused row kernel */
 #define S2T_PROF_COLS 1  /* fused column kernel */

@@ -38,10 +38,11 @@ EXPORTS = [
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
     "s2t_profile_kinds", "s2t_profile_sampled", "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
+    "s2t_column_widths", "s2t_set_column_width", "s2t_autotune",
     "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
     "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
     "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
-    "s2t_batch_sync",
+    "s2t_batch_sync", "s2t_batch_autotune", "s2t_batch_column_width",
     "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
     "s2t_server_read_state", "s2t_server_cancel", "s2t_server_set_callback", "s2t_server_drain",
     "s2t_server_get_stats",
@@ -143,6 +144,9 @@ def load(path=None):
         "s2t_profile_read": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp)]),
         "s2t_kernel_bytes": (ip, [vp, ip, ip, ctypes.POINTER(dp)]),
         "s2t_copy_probe": (ip, [dp, ip, ctypes.POINTER(dp)]),
+        "s2t_column_widths": (ip, [ip, ctypes.POINTER(ip), ip, ctypes.POINTER(ip)]),
+        "s2t_set_column_width": (ip, [vp, ip]),
+        "s2t_autotune": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp), ip]),
         "s2t_batch_create": (ip, [ctypes.POINTER(ctypes.c_int), ip, ctypes.POINTER(vp)]),
         "s2t_batch_destroy": (ip, [vp]),
         "s2t_batch_size": (ip, [vp, ip, ctypes.POINTER(ip)]),
@@ -155,6 +159,8 @@ def load(path=None):
         "s2t_batch_use_graphs": (ip, [vp, ip]),
         "s2t_batch_step": (ip, [vp, ip]),
         "s2t_batch_sync": (ip, [vp]),
+        "s2t_batch_autotune": (ip, [vp, ip]),
+        "s2t_batch_column_width": (ip, [vp, ip, ctypes.POINTER(ip)]),
         "s2t_server_create": (ip, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ip,
                                    dp, ctypes.POINTER(vp)]),
         "s2t_server_destroy": (ip, [vp]),
@@ -373,6 +379,19 @@ class Solver:
         _check(self.lib, self.lib.s2t_kernel_bytes(self.h, kind, stage, ctypes.byref(b)))
         return b.value
 
+    def set_column_width(self, width):
+        """s2t_set_column_width: a compiled column-group width for the steps (0: the default)."""
+        _check(self.lib, self.lib.s2t_set_column_width(self.h, int(width)))
+
+    def autotune(self, steps=5):
+        """s2t_autotune: (chosen width, {width: ms per step}) from a scratch solver."""
+        widths = column_widths(self.n)
+        w = ctypes.c_int()
+        ms = (ctypes.c_double * len(widths))()
+        _check(self.lib, self.lib.s2t_autotune(self.h, int(steps), ctypes.byref(w), ms,
+                                               len(widths)))
+        return w.value, dict(zip(widths, list(ms)))
+
 
 def copy_probe(nbytes, reps=10):
     """Average ms of one plain device copy of nbytes (HBM counter calibration)."""
@@ -398,6 +417,15 @@ def _diag_dict(d, spec=None, zonal=None):
         out["spectrum"] = spec
         out["zonal"] = zonal
     return out
+
+
+def column_widths(n):
+    """s2t_column_widths (host only): the compiled column-group widths for size n, default first."""
+    lib = load()
+    buf = (ctypes.c_int * 8)()
+    cnt = ctypes.c_int()
+    _check(lib, lib.s2t_column_widths(int(n), buf, 8, ctypes.byref(cnt)))
+    return list(buf[:cnt.value])
 
 
 def bandlimited_spectrum(n, kmin=8.0, kmax=16.0, amplitude=6.0, seed=2):
@@ -471,6 +499,16 @@ class Batch:
 
     def use_graphs(self, enable):
         _check(self.lib, self.lib.s2t_batch_use_graphs(self.h, 1 if enable else 0))
+
+    def autotune(self, steps=5):
+        """s2t_batch_autotune: column-group width per size class; returns {size: width}."""
+        _check(self.lib, self.lib.s2t_batch_autotune(self.h, int(steps)))
+        out = {}
+        for i, n in enumerate(self.sizes):
+            w = ctypes.c_int()
+            _check(self.lib, self.lib.s2t_batch_column_width(self.h, i, ctypes.byref(w)))
+            out[n] = w.value
+        return out
 
     def step(self, nsteps=1):
         _check(self.lib, self.lib.s2t_batch_step(self.h, int(nsteps)))

@@ -386,6 +386,37 @@ int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms) {
   S2T_API_END
 }
 
+int s2t_column_widths(int n, int* widths, int max, int* count) {
+  const Ops* ops = ops_for(n);
+  if (!ops) return api_bad(size_error(n).empty() ? "grid size " + std::to_string(n) +
+                                                       " is not supported"
+                                                 : size_error(n));
+  int w[MAX_COL_WIDTHS];
+  const int nw = ops->col_widths(w);
+  if (count) *count = nw;
+  for (int i = 0; i < nw && i < max && widths; ++i) widths[i] = w[i];
+  return S2T_OK;
+}
+
+int s2t_set_column_width(s2t_solver* h, int width) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  set_col_width(s, width);
+  S2T_API_END
+}
+
+int s2t_autotune(s2t_solver* h, int steps, int* width, double* ms_per_step, int max) {
+  SOLVER(h);
+  if (steps < 1) return api_bad("s2t_autotune: steps must be >= 1");
+  S2T_API_BEGIN
+  int w[MAX_COL_WIDTHS], nw = 0;
+  double ms[MAX_COL_WIDTHS];
+  const int best = autotune_cols(s, steps, w, ms, &nw);
+  if (width) *width = best;
+  for (int i = 0; i < nw && i < max && ms_per_step; ++i) ms_per_step[i] = ms[i];
+  S2T_API_END
+}
+
 int s2t_copy_probe(double bytes, int reps, double* ms_per_copy) {
   if (!(bytes >= 16) || reps <= 0) return api_bad("s2t_copy_probe: bytes >= 16 and reps > 0");
   void *a = nullptr, *b = nullptr;

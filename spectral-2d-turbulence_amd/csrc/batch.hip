@@ -266,6 +266,28 @@ int s2t_batch_use_graphs(s2t_batch* h, int enable) {
   S2T_API_END
 }
 
+int s2t_batch_autotune(s2t_batch* h, int steps) {
+  Batch* b = reinterpret_cast<Batch*>(h);
+  if (!b) return api_bad("s2t_batch_autotune: null batch");
+  if (steps < 1) return api_bad("s2t_batch_autotune: steps must be >= 1");
+  S2T_API_BEGIN
+  batch_quiesce(b);
+  // each class with its own member count; captured steps hold the old widths
+  for (Solver* s : b->cls) autotune_cols(s, steps, nullptr, nullptr, nullptr);
+  drop_graphs(b);
+  S2T_API_END
+}
+
+int s2t_batch_column_width(s2t_batch* h, int member, int* width) {
+  BATCH_MEMBER(h, member);
+  if (!width) return api_bad("s2t_batch_column_width: null width");
+  S2T_API_BEGIN
+  int w[MAX_COL_WIDTHS];
+  s->ops->col_widths(w);
+  *width = s->cols_c > 0 ? s->cols_c : w[0];
+  S2T_API_END
+}
+
 int s2t_batch_step(s2t_batch* h, int nsteps) {
   Batch* b = reinterpret_cast<Batch*>(h);
   if (!b) return api_bad("s2t_batch_step: null batch");

@@ -3,8 +3,10 @@
 // Solver (src/gpu/solver.js:77-608).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "aux.hpp"
@@ -371,6 +373,72 @@ void step_once(Solver* s, bool dye) {
   s->prof_skip = false;
 }
 
+// Column-group width of the step's column kernel for this solver: each compiled width is timed
+// over `steps` steps (after two untimed ones) of a scratch solver of the same size, slot count and
+// slot parameters, seeded with turbulence; the fastest is kept. The solver's own buffers are not
+// touched, and every width gives bit-identical results, so tuning never changes a result.
+// ms[i] receives each width's milliseconds per step (in the order of widths[]).
+int autotune_cols(Solver* s, int steps, int* widths, double* ms, int* count) {
+  int w[MAX_COL_WIDTHS];
+  const int nw = s->ops->col_widths(w);
+  Solver* t = solver_new(s->n, 1, s->batch, false);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int best = w[0];
+  try {
+    for (int k = 0; k < s->batch; ++k) {
+      configure_slot(t, k, s->params_slot[k]);
+      seed_spectrum_slot(t, k, std::max(2.0, s->n / 32.0), 6.0, 1u + (uint32_t)k);
+    }
+    hipchk(hipEventCreate(&e0), "hipEventCreate");
+    hipchk(hipEventCreate(&e1), "hipEventCreate");
+    double best_ms = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      t->cols_c = w[i];
+      step_once(t, false);
+      step_once(t, false);
+      hipchk(hipEventRecord(e0, t->stream), "hipEventRecord");
+      for (int k = 0; k < steps; ++k) step_once(t, false);
+      hipchk(hipEventRecord(e1, t->stream), "hipEventRecord");
+      hipchk(hipEventSynchronize(e1), "hipEventSynchronize");
+      float f = 0.f;
+      hipchk(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+      const double per = (double)f / steps;
+      if (widths) widths[i] = w[i];
+      if (ms) ms[i] = per;
+      if (i == 0 || per < best_ms) {
+        best_ms = per;
+        best = w[i];
+      }
+    }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    solver_free(t);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  solver_free(t);
+  if (count) *count = nw;
+  s->cols_c = best;
+  return best;
+}
+
+void set_col_width(Solver* s, int width) {
+  int w[MAX_COL_WIDTHS];
+  const int nw = s->ops->col_widths(w);
+  if (width == 0) {
+    s->cols_c = 0;
+    return;
+  }
+  for (int i = 0; i < nw; ++i)
+    if (w[i] == width) {
+      s->cols_c = width;
+      return;
+    }
+  throw Fail(S2T_EINVAL, "column-group width " + std::to_string(width) +
+                             " is not compiled for grid size " + std::to_string(s->n));
+}
 
 // Band-limited random vorticity (host, FP64, rounded once to FP32). Per canonical mode with
 // kmin <= |k| <= kmax in the 2/3 band: three uniforms u_j = (hash32((3 idx + j) ^ hash32(seed))

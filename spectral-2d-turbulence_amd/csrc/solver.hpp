@@ -29,7 +29,11 @@ struct Ops {
   void (*fft2)(Solver* s, float2* data, int inverse);
   void (*pack)(Solver* s, const float2* full, float2* state);
   void (*unpack)(Solver* s, const float2* state, float2* full);
+  // column-group widths compiled for the step's column kernel (the default first); returns the
+  // count, at most MAX_COL_WIDTHS
+  int (*col_widths)(int* out);
 };
+constexpr int MAX_COL_WIDTHS = 4;
 
 const Ops* ops_for(int n);
 
@@ -53,6 +57,7 @@ struct Solver {
   double2* wtab = nullptr;     // exp(2 pi i m / N), FP64, for the zonal mean
   Inst* inst = nullptr;        // per-instance parameters and scalars (batch entries)
   int batch = 1;               // instances sharing this solver's buffers
+  int cols_c = 0;              // columns per column-kernel workgroup in a step (0: the default)
   double* diag_bins = nullptr; // [2][NBINS] energy and enstrophy per bin, then 2 totals
   double* diag_zonal = nullptr;
   double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation

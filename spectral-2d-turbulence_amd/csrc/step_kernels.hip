@@ -25,8 +25,10 @@
 
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 #include "bufio.hpp"
+#include "core.hpp"
 #include "physics.hpp"
 #include "solver.hpp"
 
@@ -79,17 +81,62 @@ struct RowCfg {
 };
 template <int N>
 struct ColCfg {
-  // columns per workgroup: 2 for lines of 128+ threads (512-thread workgroups, two per CU at
-  // 64 KB of LDS each), 4 for 16..64 threads, else one 64-lane wave
-  static constexpr int C = Line<N>::T >= 128 ? 2 : (Line<N>::T >= 16 ? 4 : 64 / Line<N>::T);
+  // default columns per workgroup: 2 for lines of 64+ threads (at 4096, 512-thread workgroups,
+  // two per CU at 64 KB of LDS each; at 512 and 1024, 128-thread workgroups measured faster than
+  // 256 alone and in batches), 4 for 16..32 threads, else one 64-lane wave
+  static constexpr int C = Line<N>::T >= 64 ? 2 : (Line<N>::T >= 16 ? 4 : 64 / Line<N>::T);
   // lanes of a wave take columns in runs of RUN consecutive lanes (one column per 16- or 32-lane
   // LDS bank group for the long lines); the column buffers of the short lines are staggered
   static constexpr int RUN = Line<N>::T < 32 ? Line<N>::T : 32;
   static constexpr int LSTRIDE =
       Line<N>::LDS + (N >= 512 ? 0 : N == 256 ? 2 : N >= 64 ? 8 : 4);
-  static __device__ __forceinline__ int col(int tid) { return (tid / RUN) % C; }
-  static __device__ __forceinline__ int row(int tid) { return (tid / (RUN * C)) * RUN + tid % RUN; }
+  // column and line thread of a workgroup thread, for a workgroup of CW columns
+  template <int CW>
+  static __device__ __forceinline__ int col(int tid) { return (tid / RUN) % CW; }
+  template <int CW>
+  static __device__ __forceinline__ int row(int tid) { return (tid / (RUN * CW)) * RUN + tid % RUN; }
 };
+
+// Column-group widths (columns per workgroup) compiled for the step's column kernel, ColCfg's
+// default first; s2t_autotune picks among them per solver. Each column's arithmetic is the same
+// source at every width, and the compiled instances give bit-identical results (tested for every
+// width); the widths differ in workgroup size and count, in how many columns share a workgroup's
+// loads of a 128-byte line, and in LDS per workgroup (at most 68 KB, two workgroups per CU).
+// 4096 keeps its default only: a one-column instance fused two of the W8 products into the
+// following additions differently (results equal to FP32 rounding, not bit for bit), and it is
+// the slower width there.
+template <int N>
+struct ColWidths {
+  static constexpr int n = 1;
+  static constexpr int w[4] = {ColCfg<N>::C, 0, 0, 0};
+};
+template <> struct ColWidths<2048> { static constexpr int n = 3; static constexpr int w[4] = {2, 1, 4, 0}; };
+template <> struct ColWidths<1024> { static constexpr int n = 3; static constexpr int w[4] = {2, 4, 8, 0}; };
+template <> struct ColWidths<512>  { static constexpr int n = 3; static constexpr int w[4] = {2, 4, 8, 0}; };
+template <> struct ColWidths<256>  { static constexpr int n = 3; static constexpr int w[4] = {4, 8, 16, 0}; };
+template <> struct ColWidths<128>  { static constexpr int n = 2; static constexpr int w[4] = {8, 16, 0, 0}; };
+
+// every width: at most 1024 threads and 68 KB of LDS, and the thread mapping col<CW> / row<CW>
+// covers each (column, line thread) pair once
+template <int N>
+constexpr bool widths_fit() {
+  for (int i = 0; i < ColWidths<N>::n; ++i) {
+    const int cw = ColWidths<N>::w[i], T = Line<N>::T, run = ColCfg<N>::RUN;
+    if (cw < 1 || cw * T > 1024 || cw * ColCfg<N>::LSTRIDE * 8 > 68 * 1024) return false;
+    if (T % run != 0) return false;
+    // the thread of each (c, t), mapped back: a bijection on cw * T threads
+    for (int c = 0; c < cw; ++c)
+      for (int t = 0; t < T; ++t) {
+        const int tid = (t / run) * run * cw + c * run + t % run;
+        if (tid >= cw * T || (tid / run) % cw != c || (tid / (run * cw)) * run + tid % run != t)
+          return false;
+      }
+  }
+  return ColWidths<N>::w[0] == ColCfg<N>::C;
+}
+static_assert(widths_fit<128>() && widths_fit<256>() && widths_fit<512>() && widths_fit<1024>() &&
+                  widths_fit<2048>() && widths_fit<4096>(),
+              "column-group widths: at most 1024 threads and 68 KB of LDS, the default first");
 
 // ---------------------------------------------------------------------------------------------
 // row kernel
@@ -427,7 +474,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
   const Buf bfk = make_buf(FORCE ? a.fk + inst * a.s_state : a.fk, FORCE ? STATE_B : 0u);
   const int tid = threadIdx.x;
-  const int c = ColCfg<N>::col(tid), t = ColCfg<N>::row(tid);
+  const int c = ColCfg<N>::template col<C>(tid), t = ColCfg<N>::template row<C>(tid);
   const int kx = cg * C + c;
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
@@ -684,7 +731,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_fft_cols(float2* data, const
   using L = Line<N>;
   constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
   __shared__ float2 lds_all[C * LSTRIDE];
-  const int c = ColCfg<N>::col(threadIdx.x), t = ColCfg<N>::row(threadIdx.x);
+  const int c = ColCfg<N>::template col<C>(threadIdx.x);
+  const int t = ColCfg<N>::template row<C>(threadIdx.x);
   const int colx = blockIdx.x * C + c;
   const bool live = colx < N;
   float2* lds = lds_all + c * LSTRIDE;
@@ -819,20 +867,42 @@ struct Impl {
     return a;
   }
 
+  template <int CW>
+  static void launch_cols(Solver* s, const ColArgs& a, bool force) {
+    constexpr int GROUPS = (Ge::KC + CW - 1) / CW;
+    constexpr int G8 = (GROUPS + 7) / 8 * 8;
+    const dim3 grid(G8, s->batch);
+    if (force)
+      hipLaunchKernelGGL((k_cols<N, CW, COLS_STEP, true>), grid, dim3(CW * L::T), 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_cols<N, CW, COLS_STEP, false>), grid, dim3(CW * L::T), 0, s->stream,
+                         a);
+  }
+  template <int I = 0>
+  static void launch_cols_width(Solver* s, const ColArgs& a, bool force, int w) {
+    if constexpr (I < ColWidths<N>::n) {
+      constexpr int CW = ColWidths<N>::w[I];
+      if (w == CW) return launch_cols<CW>(s, a, force);
+      return launch_cols_width<I + 1>(s, a, force, w);
+    } else {
+      throw Fail(S2T_EINVAL, "column-group width " + std::to_string(w) +
+                                 " is not compiled for this grid size");
+    }
+  }
+
   static void cols(Solver* s, int stage, bool force, bool sd) {
     ColArgs a = col_args(s, stage);
     a.force = force ? 1 : 0;
     a.sd = sd ? 1 : 0;
-    constexpr int GROUPS = (Ge::KC + C - 1) / C;
-    constexpr int G8 = (GROUPS + 7) / 8 * 8;
-    const dim3 grid(G8, s->batch);
     prof_begin(s, S2T_PROF_COLS);
-    if (force)
-      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, true>), grid, dim3(C * L::T), 0, s->stream, a);
-    else
-      hipLaunchKernelGGL((k_cols<N, C, COLS_STEP, false>), grid, dim3(C * L::T), 0, s->stream, a);
+    launch_cols_width(s, a, force, s->cols_c > 0 ? s->cols_c : C);
     check_launch("k_cols");
     prof_end(s);
+  }
+
+  static int col_widths(int* out) {
+    for (int i = 0; i < ColWidths<N>::n; ++i) out[i] = ColWidths<N>::w[i];
+    return ColWidths<N>::n;
   }
 
   // velocity prime (encodePrime, solver.js:475-486) of instances i0 .. i0+ni-1 from state[phase]
@@ -905,8 +975,9 @@ struct Impl {
   }
 
   static const Ops* ops() {
-    static const Ops o = {Ge::STATE, Ge::PLANE, L::TWSIZE, &twiddles, &rows, &cols, &prime,
-                          &rhs,      &fft2,     &pack,     &unpack};
+    static const Ops o = {Ge::STATE, Ge::PLANE, L::TWSIZE, &twiddles, &rows,
+                          &cols,     &prime,    &rhs,      &fft2,     &pack,
+                          &unpack,   &col_widths};
     return &o;
   }
 };

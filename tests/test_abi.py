@@ -65,3 +65,17 @@ def test_bandlimited_spectrum_host_matches_oracle(s2t, n, kmin, kmax):
         s2t.bandlimited_spectrum(100, 8.0, 16.0)
     with pytest.raises(s2t.S2TError, match="kmin"):
         s2t.bandlimited_spectrum(128, 16.0, 8.0)
+
+
+def test_column_widths_host(s2t):
+    """s2t_column_widths (host only): every supported size lists its compiled column-group widths,
+    the default first, each a workgroup of at most 1024 threads (16 values per thread)."""
+    for n in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
+        w = s2t.column_widths(n)
+        assert len(w) == len(set(w)) >= 1
+        assert all(c >= 1 and c * (n // (8 if n in (32, 64, 512) else 4 if n == 16 else 16)) <= 1024
+                   for c in w)
+    assert s2t.column_widths(4096) == [2] and s2t.column_widths(1024)[0] == 2
+    assert len(s2t.column_widths(2048)) > 1
+    with pytest.raises(s2t.S2TError, match="4\\^a"):
+        s2t.column_widths(96)

@@ -129,6 +129,12 @@ class Solver {
     this.step(1);
   }
 
+  // column-group width of the column kernel, tuned on a scratch solver (s2t_autotune):
+  // {width, ms: {width: ms per step}}; results are the same at every width
+  autotune(steps = 5) {
+    return addon().autotune(this._h(), this.n, steps);
+  }
+
   sync() {
     addon().sync(this._h());
   }
@@ -344,6 +350,14 @@ class Batch {
 
   step(count = 1) {
     addon().batchStep(this.handle, count);
+  }
+
+  // column-group width per size class (s2t_batch_autotune): {n: width}
+  autotune(steps = 5) {
+    addon().batchAutotune(this.handle, steps);
+    const out = {};
+    this.sizes.forEach((n, i) => { out[n] = addon().batchColumnWidth(this.handle, i); });
+    return out;
   }
 
   sync() {

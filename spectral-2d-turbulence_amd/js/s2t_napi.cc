@@ -518,6 +518,27 @@ napi_value StepAsync(napi_env env, napi_callback_info info) {
   return queue_job(env, a.argv[0], j, "s2t_step");
 }
 
+// autotune(h, n, steps) -> {width, ms: {width: ms per step}} (s2t_autotune)
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  int widths[8], count = 0;
+  S2T_CALL(env, s2t_column_widths((int)num(env, a.argv[1], 0), widths, 8, &count));
+  int w = 0;
+  double ms[8] = {0};
+  S2T_CALL(env, s2t_autotune(h, (int)num(env, a.argv[2], 5), &w, ms, 8));
+  napi_value obj, per;
+  napi_create_object(env, &obj);
+  napi_create_object(env, &per);
+  for (int i = 0; i < count && i < 8; ++i)
+    napi_set_named_property(env, per, std::to_string(widths[i]).c_str(), make_num(env, ms[i]));
+  napi_set_named_property(env, obj, "width", make_num(env, w));
+  napi_set_named_property(env, obj, "ms", per);
+  return obj;
+}
+
 napi_value PaintDye(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 2)) return nullptr;
@@ -765,6 +786,23 @@ napi_value BatchUseGraphs(napi_env env, napi_callback_info info) {
   napi_get_value_bool(env, a.argv[1], &on);
   S2T_CALL(env, s2t_batch_use_graphs(b, on ? 1 : 0));
   return nullptr;
+}
+
+// batchAutotune(h, steps); batchColumnWidth(h, i) -> width
+napi_value BatchAutotune(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  s2t_batch* b = batch(env, a.argv[0]);
+  if (!b) return nullptr;
+  S2T_CALL(env, s2t_batch_autotune(b, (int)num(env, a.argv[1], 5)));
+  return nullptr;
+}
+
+napi_value BatchColumnWidth(napi_env env, napi_callback_info info) {
+  BATCH_MEMBER(2);
+  int w = 0;
+  S2T_CALL(env, s2t_batch_column_width(b, i, &w));
+  return make_num(env, w);
 }
 
 napi_value BatchStep(napi_env env, napi_callback_info info) {
@@ -1090,7 +1128,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"batchSeedPhysical", BatchSeedPhysical}, {"batchWriteState", BatchWriteState},
              {"batchReadState", BatchReadState}, {"batchDiagnostics", BatchDiagnostics},
              {"batchUseGraphs", BatchUseGraphs}, {"batchStep", BatchStep},
-             {"batchSync", BatchSync}};
+             {"batchSync", BatchSync}, {"batchAutotune", BatchAutotune},
+             {"batchColumnWidth", BatchColumnWidth}, {"autotune", Autotune}};
   for (const Fn& f : fns) {
     napi_value fn;
     NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

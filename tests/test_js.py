@@ -188,7 +188,18 @@ const sizes = [64, 128, 64];
   }
   let threw = false;
   try { b.readState(3); } catch (e) { threw = true; }
-  console.log(JSON.stringify({ out, threw }));
+  // tuning keeps every member's result: step on after tuning, still equal to a tuned-free solver
+  const widths = b.autotune(3);
+  b.step(2);
+  const s0 = await m.Solver.create(null, { n: 64, dyeScale: 1 });
+  s0.configure(cfgs[0]);
+  s0.seed({ k0: 6, amplitude: 5, seed: 10 });
+  const tuned = s0.autotune(3);
+  s0.step(9);
+  const x0 = b.readState(0), y0 = s0.readState();
+  let same9 = x0.length === y0.length;
+  for (let k = 0; same9 && k < x0.length; k++) same9 = x0[k] === y0[k];
+  console.log(JSON.stringify({ out, threw, widths, tuned, same9 }));
 })().catch((e) => { console.error(e); process.exit(1); });
 """
 
@@ -205,3 +216,5 @@ def test_js_batch_members_equal_plain_solvers():
     assert out["threw"]
     for rec in out["out"]:
         assert rec["same"] and rec["step"] == 7 and rec["e"] > 0
+    assert set(out["widths"]) == {"64", "128"} and out["same9"]
+    assert str(out["tuned"]["width"]) in out["tuned"]["ms"]

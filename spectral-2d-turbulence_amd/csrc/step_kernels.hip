@@ -262,7 +262,8 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
 enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2, ROWS_PAIRED_SD = 3 };
 
 template <int N, int G, int KIND>
-__global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
+__global__ void __launch_bounds__(G * Line<N>::T,
+                                  N == 4096 && KIND != ROWS_SINGLE_FORCE ? 3 : 1) k_rows(RowArgs a) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS;
@@ -296,9 +297,9 @@ __global__ void __launch_bounds__(G * Line<N>::T) k_rows(RowArgs a) {
   float* vpark = vpark_all + g * N;
   // Twiddle bases of this thread, loaded once for all transforms of the kernel: only where the
   // LDS limit (three workgroups per CU at N = 4096) leaves the registers free, in the paired
-  // variant of stages 1-2 (138 VGPRs with them; the stage-0 variants reach 170 and more, past the
-  // 168 of three waves per SIMD; smaller N would lose occupancy).
-  constexpr bool TWREG = KIND == ROWS_PAIRED && N == 4096;
+  // variants (138 VGPRs in stages 1-2; 164 at stage 0, held to the 168 of three waves per SIMD
+  // by the launch bound, without spills); smaller N would lose occupancy.
+  constexpr bool TWREG = (KIND == ROWS_PAIRED || KIND == ROWS_PAIRED_SD) && N == 4096;
   float2 tb[Bases<N>::SIZE];
   if constexpr (TWREG) load_bases<N>(tb, a.tw, t);
   auto fft_inv = [&](float2 (&x)[PTS], int tt) {

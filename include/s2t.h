@@ -133,6 +133,11 @@ int s2t_write_state(s2t_solver* h, const float* spectrum);
 int s2t_read_state(s2t_solver* h, float* spectrum);
 /* Zero the step index and time and set dt = dt_max (part of Solver.reset). */
 int s2t_reset_clock(s2t_solver* h);
+/* Set the simulated time and the step index (the stochastic forcing phases follow the step
+ * index). No counterpart in the reference, whose stepIndex is a host field (solver.js:520): with
+ * s2t_read_state / s2t_write_state and s2t_diagnostics' time and step_index it makes a
+ * checkpoint that resumes a forced run exactly. */
+int s2t_set_clock(s2t_solver* h, double time, uint32_t step_index);
 
 /* ------------------------------------------------------------------------------------------ */
 /* stepping                                                                                   */

@@ -34,7 +34,7 @@ EXPORTS = [
     "s2t_create", "s2t_destroy", "s2t_last_error", "s2t_radix_schedule", "s2t_configure",
     "s2t_nu_h", "s2t_set_splats", "s2t_seed_spectrum", "s2t_seed_physical",
     "s2t_bandlimited_spectrum", "s2t_seed_bandlimited", "s2t_write_state",
-    "s2t_read_state", "s2t_reset_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
+    "s2t_read_state", "s2t_reset_clock", "s2t_set_clock", "s2t_step", "s2t_sync", "s2t_diagnostics",
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
     "s2t_profile_kinds", "s2t_profile_sampled", "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
@@ -125,6 +125,7 @@ def load(path=None):
         "s2t_write_state": (ip, [vp, fp]),
         "s2t_read_state": (ip, [vp, fp]),
         "s2t_reset_clock": (ip, [vp]),
+        "s2t_set_clock": (ip, [vp, dp, u32]),
         "s2t_step": (ip, [vp, ip, u32]),
         "s2t_sync": (ip, [vp]),
         "s2t_diagnostics": (ip, [vp, ctypes.POINTER(Diag), fp, fp]),
@@ -287,6 +288,10 @@ class Solver:
 
     def reset_clock(self):
         _check(self.lib, self.lib.s2t_reset_clock(self.h))
+
+    def set_clock(self, time, step_index):
+        """s2t_set_clock: simulated time and step index (resume from a checkpoint)."""
+        _check(self.lib, self.lib.s2t_set_clock(self.h, float(time), int(step_index) & 0xFFFFFFFF))
 
     def step(self, nsteps=1, dye=False):
         _check(self.lib, self.lib.s2t_step(self.h, int(nsteps), S2T_STEP_DYE if dye else 0))

@@ -2,6 +2,7 @@
 // src/gpu/solver.js; the batch and serving entry points are in batch.hip.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -177,6 +178,14 @@ int s2t_reset_clock(s2t_solver* h) {
   S2T_API_BEGIN
   reset_slot(s, 0);
   sync(s);
+  S2T_API_END
+}
+
+int s2t_set_clock(s2t_solver* h, double time, uint32_t step_index) {
+  SOLVER(h);
+  if (!std::isfinite(time)) return api_bad("s2t_set_clock: time must be finite");
+  S2T_API_BEGIN
+  set_clock_slot(s, 0, time, step_index);
   S2T_API_END
 }
 

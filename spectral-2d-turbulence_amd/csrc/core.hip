@@ -265,6 +265,13 @@ void reset_slot(Solver* s, int slot) {
   s->primed_slot[slot] = 0;
 }
 
+// simulated time and step index of a slot (the forcing phases follow the step index), for resuming
+// from a checkpoint; waits for queued work first
+void set_clock_slot(Solver* s, int slot, double t, unsigned step) {
+  h2d(s, &s->inst[slot].st.t, &t, sizeof(double));
+  h2d(s, &s->inst[slot].st.step, &step, sizeof(unsigned));
+}
+
 void set_budget_slot(Solver* s, int slot, int steps) {
   h2d(s, &s->inst[slot].st.steps_left, &steps, sizeof(int));
 }

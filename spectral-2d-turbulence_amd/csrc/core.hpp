@@ -42,6 +42,7 @@ void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zon
                int phase = -1);
 // limit the slot to `steps` more steps, after which it retires itself (steps < 0: no limit)
 void set_budget_slot(Solver* s, int slot, int steps);
+void set_clock_slot(Solver* s, int slot, double t, unsigned step);
 DevState read_clock_slot(Solver* s, int slot);
 
 void ensure_primed(Solver* s);

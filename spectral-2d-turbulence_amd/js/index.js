@@ -171,6 +171,13 @@ class Solver {
     addon().writeState(this._h(), f, this.n);
   }
 
+  // simulated time and step index (resuming a checkpoint of readState() and the diagnostics'
+  // time and stepIndex: the forcing phases follow the step index)
+  setClock({ time = 0, stepIndex = 0 } = {}) {
+    this.simTime = time;
+    addon().setClock(this._h(), time, stepIndex);
+  }
+
   // the pointwise stages on host arrays (reference harness, tests/index.html:161-182)
   velocityPack(spectrum) {
     const f = spectrum instanceof Float32Array ? spectrum : Float32Array.from(spectrum);

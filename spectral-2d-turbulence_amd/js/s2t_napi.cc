@@ -345,6 +345,17 @@ napi_value ResetClock(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setClock(h, time, stepIndex) (s2t_set_clock)
+napi_value SetClock(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  S2T_CALL(env, s2t_set_clock(h, num(env, a.argv[1], 0),
+                              (uint32_t)(int64_t)num(env, a.argv[2], 0)));
+  return nullptr;
+}
+
 napi_value Step(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
@@ -1129,7 +1140,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"batchReadState", BatchReadState}, {"batchDiagnostics", BatchDiagnostics},
              {"batchUseGraphs", BatchUseGraphs}, {"batchStep", BatchStep},
              {"batchSync", BatchSync}, {"batchAutotune", BatchAutotune},
-             {"batchColumnWidth", BatchColumnWidth}, {"autotune", Autotune}};
+             {"batchColumnWidth", BatchColumnWidth}, {"autotune", Autotune},
+             {"setClock", SetClock}};
   for (const Fn& f : fns) {
     napi_value fn;
     NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

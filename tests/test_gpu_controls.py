@@ -123,3 +123,37 @@ def test_seed_bandlimited_stores_the_host_spectrum(s2t, n):
         assert d["stepIndex"] == 0 and d["time"] == 0.0
         s.step(1)
         assert s.diagnostics()["stepIndex"] == 1
+
+
+def test_checkpoint_resumes_a_forced_run(s2t):
+    """A checkpoint is the state (s2t_read_state) and the clock (time and step index from the
+    diagnostics); written into a new solver with s2t_write_state and s2t_set_clock, the run goes
+    on with the same stochastic forcing phases and CFL steps as the run that never stopped."""
+    n = 256
+    phys = dict(PHYS, force_amp=4.0, force_k=20.0, force_width=2.5)
+    with s2t.Solver(n) as a:
+        a.configure(**phys)
+        a.seed_spectrum(k0=12.0, amplitude=5.0, seed=21)
+        a.step(4)
+        ck_state, ck = a.read_state(), a.diagnostics()
+        a.step(6)
+        want, dwant = a.read_state(), a.diagnostics()
+    with s2t.Solver(n) as b:
+        b.configure(**phys)
+        b.write_state(ck_state)
+        b.set_clock(ck["time"], ck["stepIndex"])
+        d = b.diagnostics()
+        assert d["stepIndex"] == 4 and d["time"] == ck["time"]
+        b.step(6)
+        got, dgot = b.read_state(), b.diagnostics()
+    assert dgot["stepIndex"] == dwant["stepIndex"] == 10
+    assert dgot["time"] == pytest.approx(dwant["time"], rel=1e-12)
+    # the resumed run re-primes its velocity planes from the written state with the prime
+    # kernel, which the uninterrupted run took from its last column kernel: FP32 round-off only
+    assert np.max(np.abs(got - want)) <= 1e-6 * np.max(np.abs(want))
+    # without the clock the forcing phases restart at step 0 and the runs part
+    with s2t.Solver(n) as c:
+        c.configure(**phys)
+        c.write_state(ck_state)
+        c.step(6)
+        assert np.max(np.abs(c.read_state() - want)) > 1e-3 * np.max(np.abs(want))

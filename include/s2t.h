@@ -242,6 +242,7 @@ typedef struct s2t_server s2t_server;
 #define S2T_REQ_DONE 2
 #define S2T_REQ_REJECTED 3
 #define S2T_REQ_CANCELLED 4
+#define S2T_REQ_FAILED 5 /* the server's worker stopped on an error; it accepts nothing more */
 
 typedef struct s2t_request {
   int n;                 /* grid size; must be one of the server's classes */
@@ -279,8 +280,9 @@ int s2t_server_read_state(s2t_server* s, uint64_t id, float* spectrum);
  * S2T_ESTATE if the request has already finished (done, rejected or cancelled). */
 int s2t_server_cancel(s2t_server* s, uint64_t id);
 /* Completion callback, called once per request when it reaches a final status (done, rejected,
- * cancelled): on the worker thread, or on the submitting / cancelling thread when the status is
- * final at that call. It runs without the server's lock held, so it may call s2t_server_poll and
+ * cancelled, failed): on the worker thread, or on the submitting / cancelling thread when the
+ * status is final at that call. If the worker stops on an error, every request still queued or
+ * running fails (S2T_REQ_FAILED) and is called back; s2t_server_poll then reports the error. It runs without the server's lock held, so it may call s2t_server_poll and
  * s2t_server_read_state; it must not call s2t_server_destroy. NULL removes it. */
 typedef void (*s2t_server_callback)(uint64_t id, int status, void* user);
 int s2t_server_set_callback(s2t_server* s, s2t_server_callback cb, void* user);

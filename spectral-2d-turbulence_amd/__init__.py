@@ -48,7 +48,7 @@ EXPORTS = [
     "s2t_server_get_stats",
 ]
 SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
-REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED = 0, 1, 2, 3, 4
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4, 5
 
 
 class Params(ctypes.Structure):

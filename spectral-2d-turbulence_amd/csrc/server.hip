@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -128,6 +129,9 @@ struct Server {
   double t_first = 0, t_last = 0;
   uint64_t submitted = 0, completed = 0, rejected = 0, cancelled = 0;
   std::string fatal;
+  // failure injection for tests of the failure path, from the environment at creation
+  // (S2T_SERVER_FAIL_AFTER = k: the worker fails at the admission after k admissions)
+  int fail_after = -1, admitted = 0;
   s2t_server_callback callback = nullptr;
   void* callback_user = nullptr;
 };
@@ -278,10 +282,13 @@ static void worker_loop(Server* sv) {
       }
       for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
       for (auto& tr : take) {
+        if (sv->fail_after >= 0 && sv->admitted >= sv->fail_after)
+          throw Fail(S2T_EHIP, "worker failure injected by S2T_SERVER_FAIL_AFTER");
         Class& c = sv->cls[tr.first];
         int slot = 0;
         while (c.slot_req[slot] >= 0) ++slot;
         admit(sv, c, slot, tr.second);
+        sv->admitted += 1;
       }
       // cancellations of running requests, between steps (every class stream is idle here)
       std::vector<uint64_t> stopped;
@@ -332,8 +339,22 @@ static void worker_loop(Server* sv) {
       }
     }
   } catch (const std::exception& e) {
-    std::lock_guard<std::mutex> lk(sv->mu);
-    sv->fatal = e.what();
+    // the worker stops: every request still queued, being admitted or running fails, and its
+    // callback says so, so that no caller waits for it
+    std::vector<uint64_t> failed;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->fatal = e.what();
+      for (auto& kv : sv->rec)
+        if (kv.second.status == S2T_REQ_QUEUED || kv.second.status == S2T_REQ_RUNNING) {
+          kv.second.status = S2T_REQ_FAILED;
+          kv.second.t_done = now_s();
+          failed.push_back(kv.first);
+        }
+      sv->pending.clear();
+    }
+    std::sort(failed.begin(), failed.end());
+    for (uint64_t id : failed) notify(sv, id, S2T_REQ_FAILED);
   }
 }
 
@@ -387,6 +408,7 @@ int s2t_server_create(const int* sizes, const int* slots, int nclasses, double b
   Server* sv = new Server();
   S2T_API_BEGIN
   sv->budget = (size_t)budget_bytes;
+  if (const char* f = std::getenv("S2T_SERVER_FAIL_AFTER")) sv->fail_after = std::atoi(f);
   for (int i = 0; i < nclasses; ++i) {
     if (sv->cls.count(sizes[i])) throw Fail(S2T_EINVAL, "duplicate class size");
     Class c;

@@ -386,14 +386,15 @@ class Batch {
 }
 
 // request statuses and server-side scenarios of include/s2t.h
-const REQ = Object.freeze({ QUEUED: 0, RUNNING: 1, DONE: 2, REJECTED: 3, CANCELLED: 4 });
+const REQ = Object.freeze({ QUEUED: 0, RUNNING: 1, DONE: 2, REJECTED: 3, CANCELLED: 4, FAILED: 5 });
 const SERVER_SCENARIOS = Object.freeze({ turbulence: 0, 'taylor-green': 1, dipole: 2, merger: 3 });
 
-// Serving loop (include/s2t.h s2t_server_*): requests (a scenario, a grid size and a step count) share pre-allocated solver
-// slots per grid size and run on the library's worker thread. submit() returns the request id at
-// once and a Promise that settles when the request reaches its final status: resolved with the
-// FP64 diagnostics (and the final spectrum with keepState) when done, rejected with an Error
-// whose `status` is REQ.REJECTED or REQ.CANCELLED otherwise.
+// Serving loop (include/s2t.h s2t_server_*): requests (a scenario, a grid size and a step count)
+// share pre-allocated solver slots per grid size and run on the library's worker thread. submit()
+// returns the request id at once and a Promise that settles when the request reaches its final
+// status: resolved with the FP64 diagnostics (and the final spectrum with keepState) when done,
+// rejected with an Error whose `status` is REQ.REJECTED, REQ.CANCELLED or REQ.FAILED otherwise
+// (FAILED: the server's worker stopped on an error, whose text is in the message).
 class Server {
   // classes: [{n, slots}], budgetBytes: device memory limit for the slots (0 = none)
   constructor(classes, { budgetBytes = 0 } = {}) {
@@ -454,7 +455,7 @@ class Server {
       s.resolve(result);
     } else {
       const e = new Error(status === REQ.CANCELLED ? `request ${id} cancelled`
-        : `request ${id} rejected`);
+        : status === REQ.FAILED ? `request ${id} failed: ${result}` : `request ${id} rejected`);
       e.status = status;
       s.reject(e);
     }

@@ -854,6 +854,7 @@ struct Final {
   s2t_diag diag{};
   double latency = -1;
   std::vector<float> state;
+  std::string error;  // the server's error when the request failed
 };
 
 // s2t_server_callback: runs on the server's worker thread (or on the submitting / cancelling
@@ -869,6 +870,10 @@ void server_final(uint64_t id, int status, void* user) {
       n = it->second;
       b->keep.erase(it);
     }
+  }
+  if (status == S2T_REQ_FAILED) {
+    int st = 0;
+    if (s2t_server_poll(b->sv, id, &st, nullptr, nullptr) != S2T_OK) f->error = s2t_last_error();
   }
   if (status == S2T_REQ_DONE) {
     int st = 0;
@@ -896,7 +901,8 @@ napi_value diag_object(napi_env env, int status, const s2t_diag& d, double laten
 }
 
 // calls onFinal(id, status, result): result is the diagnostics object (with `state`, a
-// Float32Array in fixture layout, for keepState) when done, else undefined
+// Float32Array in fixture layout, for keepState) when done, the server's error text when failed,
+// else undefined
 void final_to_js(napi_env env, napi_value js_cb, void*, void* data) {
   Final* f = static_cast<Final*>(data);
   if (env && js_cb) {
@@ -905,6 +911,8 @@ void final_to_js(napi_env env, napi_value js_cb, void*, void* data) {
     napi_create_double(env, (double)f->id, &args[0]);
     napi_create_int32(env, f->status, &args[1]);
     args[2] = undef;
+    if (f->status == S2T_REQ_FAILED)
+      napi_create_string_utf8(env, f->error.c_str(), NAPI_AUTO_LENGTH, &args[2]);
     if (f->status == S2T_REQ_DONE) {
       args[2] = diag_object(env, f->status, f->diag, f->latency);
       if (!f->state.empty()) {

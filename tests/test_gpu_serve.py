@@ -116,3 +116,34 @@ def test_cancel_and_completion_callbacks(s2t):
         s.seed_spectrum(k0=6.0, amplitude=5.0, seed=11)
         s.step(7)
         assert np.array_equal(got, s.read_state())
+
+
+def test_worker_failure_fails_every_outstanding_request(s2t, monkeypatch):
+    """If the worker stops on an error (forced from the host: S2T_SERVER_FAIL_AFTER = 1 makes the
+    second admission fail), every request still queued or running fails and is called back once,
+    so no caller waits forever; the server then accepts nothing more and reports the error."""
+    import threading
+    seen, lock, done = {}, threading.Lock(), threading.Event()
+
+    def on_final(rid, status):
+        with lock:
+            assert rid not in seen
+            seen[rid] = status
+            if len(seen) == 3:
+                done.set()
+
+    monkeypatch.setenv("S2T_SERVER_FAIL_AFTER", "1")
+    sv = s2t.Server({64: 1})
+    monkeypatch.delenv("S2T_SERVER_FAIL_AFTER")
+    try:
+        sv.set_callback(on_final)
+        ids = [sv.submit(n=64, steps=20, seed=k, **PHYS) for k in range(3)]
+        assert done.wait(60)
+        # one slot: the first request runs to completion, the second admission fails
+        assert [seen[i] for i in ids] == [s2t.REQ_DONE, s2t.REQ_FAILED, s2t.REQ_FAILED]
+        with pytest.raises(s2t.S2TError, match="injected"):
+            sv.submit(n=64, steps=5, **PHYS)
+        with pytest.raises(s2t.S2TError, match="injected"):
+            sv.drain(1)
+    finally:
+        sv.close()

@@ -218,3 +218,32 @@ def test_js_batch_members_equal_plain_solvers():
         assert rec["same"] and rec["step"] == 7 and rec["e"] > 0
     assert set(out["widths"]) == {"64", "128"} and out["same9"]
     assert str(out["tuned"]["width"]) in out["tuned"]["ms"]
+
+
+FAIL_JS = r"""
+const m = require('./index.js');
+const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };
+(async () => {
+  const sv = new m.Server([{ n: 64, slots: 1 }]);
+  const reqs = [0, 1, 2].map((k) => sv.submit({ n: 64, steps: 20, seed: k }, phys));
+  const out = await Promise.all(reqs.map((r) => r.result.then(
+    (d) => ({ status: d.status }), (e) => ({ status: e.status, message: e.message }))));
+  sv.destroy();
+  console.log(JSON.stringify(out));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_server_worker_failure_settles_every_promise():
+    """A worker failure (forced from the host with S2T_SERVER_FAIL_AFTER = 1: the second admission
+    fails) rejects every outstanding Promise with REQ.FAILED and the error text, and the process
+    ends by itself."""
+    import json
+    env = dict(os.environ, S2T_SERVER_FAIL_AFTER="1")
+    r = subprocess.run([NODE, "-e", FAIL_JS], cwd=JS, capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert [o["status"] for o in out] == [2, 5, 5]
+    assert all("injected" in o["message"] for o in out[1:])

@@ -128,6 +128,15 @@ __device__ __forceinline__ void dft16(float2* v) {
     for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[4 * k1 + k2];
 }
 
+// Order in which to load a thread's R values for the first DFT: the order its first butterflies
+// consume them (radix 16: the four DFT4s over n2 take v[n2], v[4 + n2], v[8 + n2], v[12 + n2];
+// radix 8: the even, then the odd DFT4). Memory waits are in issue order, so the first butterfly
+// can start once its own inputs have arrived.
+template <int R>
+constexpr int dft_load_order(int i) {
+  return R == 16 ? (i % 4) * 4 + i / 4 : R == 8 ? (i % 4) * 2 + i / 4 : i;
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* v) {
   if constexpr (R == 2) dft2<DIR>(v[0], v[1]);

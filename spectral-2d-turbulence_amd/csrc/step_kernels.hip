@@ -224,7 +224,8 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
     }
   };
 #pragma unroll
-  for (int q = 0; q < PTS; ++q) {
+  for (int i = 0; i < PTS; ++i) {
+    const int q = dft_load_order<PTS>(i);  // the first DFT's order of use
     const int k = t + q * T;
     float2 v = make_float2(0.f, 0.f);
     if (2 * q < PTS) {
@@ -483,6 +484,8 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   // per-lane byte offsets: row t of column kx in a 2-row-tiled mixed plane / in the compact state
   // (row block q adds q T rows: q T / 2 row pairs, T is even)
   const uint32_t vm = Ge::off(t, kx);
+  // the same, out of range for a lane past the last column (its loads read zero, with no branch)
+  const uint32_t vmp = active ? vm : BUF_OOB;
   const uint32_t vs = ((uint32_t)kx * Ge::PITCH + (uint32_t)t) * 8u;
   // row block q holds rows t + q T: the low half are the compact rows j = i, the high half
   // j = i - (N - NJ) (T divides N / 2, so a block never spans both halves)
@@ -529,8 +532,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       if (MODE == COLS_RHS || a.stage == 0) {
         float2 F[PTS];
 #pragma unroll
-        for (int q = 0; q < PTS; ++q)
-          F[q] = active ? ld2(bf, vm, q * ROWS_B) : make_float2(0.f, 0.f);
+        for (int i = 0; i < PTS; ++i) {
+          const int q = dft_load_order<PTS>(i);
+          F[q] = ld2(bf, vmp, q * ROWS_B);
+        }
         line_fft<N, -1>(F, lds, t, a.tw);
 #pragma unroll
         for (int q = 0; q < PTS; ++q)
@@ -540,8 +545,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
     {
       float2 P[PTS];
 #pragma unroll
-      for (int q = 0; q < PTS; ++q)
-        P[q] = active ? ld2(bz, vm, q * ROWS_B) : make_float2(0.f, 0.f);
+      for (int i = 0; i < PTS; ++i) {
+        const int q = dft_load_order<PTS>(i);
+        P[q] = ld2(bz, vmp, q * ROWS_B);
+      }
       line_fft<N, -1>(P, lds, t, a.tw);
 #pragma unroll
       for (int q = 0; q < PTS; ++q)
@@ -551,8 +558,10 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       // Q_hat parks in the exchange buffer, each thread at its own indices
       float2 Q[PTS];
 #pragma unroll
-      for (int q = 0; q < PTS; ++q)
-        Q[q] = active ? ld2(bz, vm, PLANE_B + q * ROWS_B) : make_float2(0.f, 0.f);
+      for (int i = 0; i < PTS; ++i) {
+        const int q = dft_load_order<PTS>(i);
+        Q[q] = ld2(bz, vmp, PLANE_B + q * ROWS_B);
+      }
       line_fft<N, -1>(Q, lds, t, a.tw);
       __syncthreads();
 #pragma unroll

@@ -79,3 +79,30 @@ def test_column_widths_host(s2t):
     assert len(s2t.column_widths(2048)) > 1
     with pytest.raises(s2t.S2TError, match="4\\^a"):
         s2t.column_widths(96)
+
+
+def test_argument_checks_before_any_device_work(s2t):
+    """Argument errors of the batch, server and solver entry points are found on the host (no GPU
+    call is made), with the messages a caller sees."""
+    import ctypes
+    lib = s2t.load()
+    h = ctypes.c_void_p()
+    empty = (ctypes.c_int * 1)(0)
+    assert lib.s2t_batch_create(empty, 0, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert "at least one member" in lib.s2t_last_error().decode()
+    ragged = (ctypes.c_int * 3)(256, 100, 512)
+    assert lib.s2t_batch_create(ragged, 3, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert "4^a" in lib.s2t_last_error().decode() and not h.value
+    sizes, slots = (ctypes.c_int * 2)(256, 512), (ctypes.c_int * 2)(4, 0)
+    assert lib.s2t_server_create(sizes, slots, 2, 0.0, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert "slot counts" in lib.s2t_last_error().decode()
+    slots = (ctypes.c_int * 2)(4, 4)
+    assert lib.s2t_server_create(sizes, slots, 0, 0.0, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert lib.s2t_server_create(sizes, slots, 2, 1e6, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert "over the budget" in lib.s2t_last_error().decode()
+    # null handles everywhere
+    assert lib.s2t_batch_step(None, 1) == s2t.S2T_EINVAL
+    assert lib.s2t_server_submit(None, None, None) == s2t.S2T_EINVAL
+    assert lib.s2t_autotune(None, 3, None, None, 0) == s2t.S2T_EINVAL
+    assert lib.s2t_set_clock(None, 0.0, 0) == s2t.S2T_EINVAL
+    assert lib.s2t_profile_sampled(None, 1, 4) == s2t.S2T_EINVAL

@@ -157,3 +157,19 @@ def test_checkpoint_resumes_a_forced_run(s2t):
         c.write_state(ck_state)
         c.step(6)
         assert np.max(np.abs(c.read_state() - want)) > 1e-3 * np.max(np.abs(want))
+
+
+def test_zero_and_negative_step_counts(s2t):
+    """step(0) changes nothing (state, clock, diagnostics); a negative count is an error."""
+    with s2t.Solver(128) as s:
+        s.configure(**PHYS)
+        s.seed_spectrum(k0=8.0, amplitude=5.0, seed=3)
+        s.step(2)
+        before, d0 = s.read_state(), s.diagnostics()
+        s.step(0)
+        assert np.array_equal(s.read_state(), before)
+        d1 = s.diagnostics()
+        assert (d1["stepIndex"], d1["time"], d1["energy"]) == (d0["stepIndex"], d0["time"],
+                                                               d0["energy"])
+        with pytest.raises(s2t.S2TError, match="nsteps"):
+            s.step(-1)

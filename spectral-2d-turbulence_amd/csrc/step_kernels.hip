@@ -168,7 +168,8 @@ static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() ==
               "4096: blocks 6..9 are out of band");
 
 // Hermitian unpack along x of a forward row transform z held by the line (thread t holds
-// z[t + q T]): the two real fields packed as re / im, for kx = 0..K, into 2-row-tiled planes.
+// z[t + q T]): the two real fields packed as re / im, for kx = 0..K, into the tiled planes
+// (Geo::off).
 // `voff` is the lane's byte offset of (row pair, kx = t). PAIR: the fields are rows 2m and 2m+1
 // of one plane, stored together as one 16-byte element; otherwise they are one row of two planes
 // (uniform byte offsets soff0 / soff1, which include the row's 8-byte slot in the pair).
@@ -206,7 +207,7 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
 
 // Inverse row transform input, Hermitian-extended to the full row: thread t, value q holds
 // frequency k = t + q T. vlo / vhi are the lane's byte offsets of (row pair, kx = t) and
-// (row pair, kx = T - t) in a 2-row-tiled plane. VPAIR: x = a(y0) + i a(y0+1) for one real field
+// (row pair, kx = T - t) in a tiled plane. VPAIR: x = a(y0) + i a(y0+1) for one real field
 // a (plane offset soffa; one 16-byte load per value); otherwise x = a(y) + i b(y) for two real
 // fields of one row (uniform offsets soffa / soffb include the row's 8-byte slot).
 template <int N, bool VPAIR>
@@ -485,7 +486,7 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
   const float fkx = (float)kx;
-  // per-lane byte offsets: row t of column kx in a 2-row-tiled mixed plane / in the compact state
+  // per-lane byte offsets: row t of column kx in a tiled mixed plane / in the compact state
   // (row block q adds q T rows: q T / 2 row pairs, T is even)
   const uint32_t vm = Ge::off(t, kx);
   // the same, out of range for a lane past the last column (its loads read zero, with no branch)

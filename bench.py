@@ -130,6 +130,7 @@ def bench_ensemble(s2t, steps, warmup, plain_steps):
             b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=1000 + i)
         b.step(warmup)
         b.sync()
+        default_widths = b.column_widths()
         # the default column-group widths, then the widths the tuner picks for each size class
         # with its member count (tuning runs on scratch solvers, outside the timed regions)
         t0 = time.perf_counter()
@@ -169,8 +170,7 @@ def bench_ensemble(s2t, steps, warmup, plain_steps):
                        "members": len(sizes), "points": pts,
                        "column_widths": {str(k): v for k, v in sorted(widths.items())}},
             "default_widths": {"value": round(pts * steps / el_default / 1e6, 2),
-                               "widths": {str(n): s2t.column_widths(n)[0]
-                                          for n in sorted(set(sizes))}},
+                               "widths": {str(k): v for k, v in sorted(default_widths.items())}},
             "plain_loop": {"value": round(plain_rate, 2), "steps_per_member": plain_steps,
                            "seconds": round(el_plain, 3)},
             "speedup_vs_plain_loop": round(batch_rate / plain_rate, 3)}

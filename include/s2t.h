@@ -221,7 +221,8 @@ int s2t_batch_sync(s2t_batch* b);
 /* s2t_autotune for every size class of the batch, each with its own member count (replaces the
  * captured steps). */
 int s2t_batch_autotune(s2t_batch* b, int steps);
-/* Column-group width in effect for a member's size class. */
+/* Column-group width in effect for a member's size class: the tuned width, else 4 for a class of
+ * two or more members where 4 is compiled, else the size's default (s2t_column_widths, first). */
 int s2t_batch_column_width(s2t_batch* b, int member, int* width);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -508,6 +508,10 @@ class Batch:
     def autotune(self, steps=5):
         """s2t_batch_autotune: column-group width per size class; returns {size: width}."""
         _check(self.lib, self.lib.s2t_batch_autotune(self.h, int(steps)))
+        return self.column_widths()
+
+    def column_widths(self):
+        """s2t_batch_column_width of each size class; returns {size: width}."""
         out = {}
         for i, n in enumerate(self.sizes):
             w = ctypes.c_int()

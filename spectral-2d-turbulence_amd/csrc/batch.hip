@@ -165,6 +165,7 @@ int s2t_batch_create(const int* sizes, int count, s2t_batch** out) {
   for (auto& kv : per_size) {
     cls_of[kv.first] = (int)b->cls.size();
     b->cls.push_back(solver_new(kv.first, 1, kv.second, false));
+    default_class_width(b->cls.back(), kv.second);
     next_slot[kv.first] = 0;
   }
   for (int i = 0; i < count; ++i) b->member.push_back({cls_of[sizes[i]], next_slot[sizes[i]]++});

@@ -431,6 +431,17 @@ int autotune_cols(Solver* s, int steps, int* widths, double* ms, int* count) {
   return best;
 }
 
+// Default width of a batch class: a class of several members fills the chip by itself, so it takes
+// 4 columns per workgroup where that width is compiled (the width the tuner chose for every
+// multi-member class measured, 256^2 to 2048^2), else the size's single-instance default.
+void default_class_width(Solver* s, int members) {
+  if (members < 2) return;
+  int w[MAX_COL_WIDTHS];
+  const int nw = s->ops->col_widths(w);
+  for (int i = 0; i < nw; ++i)
+    if (w[i] == 4) s->cols_c = 4;
+}
+
 void set_col_width(Solver* s, int width) {
   int w[MAX_COL_WIDTHS];
   const int nw = s->ops->col_widths(w);

@@ -82,11 +82,10 @@ struct RowCfg {
 template <int N>
 struct ColCfg {
   // default columns per workgroup, the fastest single instance measured per line length: 2 at
-  // 256 threads (4096: 512-thread workgroups, two per CU at 64 KB of LDS each), 1 at 128 (2048),
-  // 2 at 64 (512, 1024), 4 at 16..32, else one 64-lane wave
+  // 256 threads (4096: 512-thread workgroups, two per CU at 64 KB of LDS each), 1 at 64..128
+  // (512, 1024, 2048), 4 at 16..32, else one 64-lane wave
   static constexpr int C = Line<N>::T >= 256  ? 2
-                           : Line<N>::T >= 128 ? 1
-                           : Line<N>::T >= 64  ? 2
+                           : Line<N>::T >= 64  ? 1
                            : Line<N>::T >= 16  ? 4
                                                : 64 / Line<N>::T;
   // lanes of a wave take columns in runs of RUN consecutive lanes (one column per 16- or 32-lane
@@ -115,8 +114,8 @@ struct ColWidths {
   static constexpr int w[4] = {ColCfg<N>::C, 0, 0, 0};
 };
 template <> struct ColWidths<2048> { static constexpr int n = 3; static constexpr int w[4] = {1, 2, 4, 0}; };
-template <> struct ColWidths<1024> { static constexpr int n = 3; static constexpr int w[4] = {2, 4, 8, 0}; };
-template <> struct ColWidths<512>  { static constexpr int n = 3; static constexpr int w[4] = {2, 4, 8, 0}; };
+template <> struct ColWidths<1024> { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
+template <> struct ColWidths<512>  { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
 template <> struct ColWidths<256>  { static constexpr int n = 3; static constexpr int w[4] = {4, 8, 16, 0}; };
 template <> struct ColWidths<128>  { static constexpr int n = 2; static constexpr int w[4] = {8, 16, 0, 0}; };
 

@@ -75,7 +75,7 @@ def test_column_widths_host(s2t):
         assert len(w) == len(set(w)) >= 1
         assert all(c >= 1 and c * (n // (8 if n in (32, 64, 512) else 4 if n == 16 else 16)) <= 1024
                    for c in w)
-    assert s2t.column_widths(4096) == [2] and s2t.column_widths(1024)[0] == 2
+    assert s2t.column_widths(4096) == [2] and s2t.column_widths(1024)[0] == 1
     assert len(s2t.column_widths(2048)) > 1
     with pytest.raises(s2t.S2TError, match="4\\^a"):
         s2t.column_widths(96)

@@ -69,3 +69,11 @@ def test_batch_autotune_members_equal_plain_solvers(s2t):
     for g, n in zip(got, (256, 1024)):
         ref, _ = run(s2t, n, 0)
         assert np.array_equal(g, ref)
+
+
+def test_batch_default_widths(s2t):
+    """A size class of two or more members defaults to 4 columns per workgroup where that width is
+    compiled; a single member keeps the size's single-instance default (the first listed)."""
+    sizes = [512] * 2 + [1024] + [2048] * 2 + [128] * 3
+    with s2t.Batch(sizes) as b:
+        assert b.column_widths() == {512: 4, 1024: s2t.column_widths(1024)[0], 2048: 4, 128: 8}

@@ -12,6 +12,7 @@ four (all of its stages); `cpu_baseline` times the FP64 NumPy oracle step on the
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-cpu-baseline]
     python bench.py --workload ensemble [--steps K] [--plain-steps P]
     python bench.py --workload serve [--rates r1,r2,...] [--requests R]
+    python bench.py --workload decaying [--steps K]
 """
 
 import argparse
@@ -232,6 +233,69 @@ def bench_serve(s2t, rates, count, seed=5):
             "sweep": lines}
 
 
+def blob_dye(m, seed, count=12):
+    """Seeded Gaussian colour blobs on the m x m dye grid (rgba16f, alpha 1, colours in [0, 1])."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(m) + 0.5) / m
+    dye = np.zeros((m, m, 4), np.float32)
+    dye[..., 3] = 1.0
+    for _ in range(count):
+        cx, cy, r = rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0.03, 0.1)
+        dx = (x - cx + 0.5) % 1.0 - 0.5
+        dy = (x - cy + 0.5) % 1.0 - 0.5
+        g = np.exp(-(dy[:, None] ** 2 + dx[None, :] ** 2) / r ** 2).astype(np.float32)
+        dye[..., :3] += g[..., None] * rng.uniform(0, 1, 3).astype(np.float32)
+    dye[..., :3] = np.clip(dye[..., :3], 0.0, 1.0)
+    return dye.astype(np.float16)
+
+
+def bench_decaying(s2t, steps, warmup):
+    """Decaying turbulence at 1024^2 with dye on the 2048^2 refined grid: band-limited seed
+    |k| in [8, 16] (seed 2, omega_rms 6), hyperviscosity p = 4 at rate omega_rms, CFL 0.5, seeded
+    Gaussian dye blobs, BFECC dye advection every step. The step rate is timed without events;
+    the dye step and the column kernel are then timed with events in a separate run."""
+    n = 1024
+    params = dict(beta=0.0, drag=0.0, viscosity=6.0, hyper_p=4.0, force_amp=0.0, force_k=32.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    with s2t.Solver(n, dye_scale=2) as s:
+        m = s.dye_size()
+        s.configure(**params)
+        s.seed_bandlimited(8.0, 16.0, 6.0, seed=2)
+        s.write_dye(blob_dye(m, seed=2))
+        s.step(warmup, dye=True)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps, dye=True)
+        s.sync()
+        el = time.perf_counter() - t0
+        s.profile(True, kinds=[s2t.PROF_DYE, s2t.PROF_COLS, s2t.PROF_ROWS])
+        s.step(max(10, steps // 4), dye=True)
+        s.sync()
+        s.profile(False)
+        prof = {k: s.profile_read(k) for k in (s2t.PROF_ROWS, s2t.PROF_COLS, s2t.PROF_DYE)}
+        dye_bytes = s.kernel_bytes(s2t.PROF_DYE)
+        d = s.diagnostics()
+        if not np.isfinite(d["energy"]) or d["nonfinite"]:
+            raise RuntimeError("non-finite state after the timed steps")
+    avg_us = {k: prof[k][1] / max(1, prof[k][0]) * 1e3 for k in prof}
+    dye_gbs = dye_bytes / (avg_us[s2t.PROF_DYE] * 1e-6) / 1e9
+    return {"metric": "Mpts·RK3-steps/s at 1024² FP32 with dye at 2048²",
+            "value": round(n * n * steps / el / 1e6, 2), "unit": "Mpts·steps/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": round(el * 1e3 / steps, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: band-limited seed 2, seeded Gaussian dye blobs",
+            "config": {"workload": "decaying turbulence 1024^2 FP32, dye 2048^2 rgba16f", "n": n,
+                       "dye": m, "hyper_p": 4, "viscosity_rate": 6.0, "cfl": 0.5,
+                       "final_dt": d["dt"], "final_energy": d["energy"]},
+            "dye_step": {"bound": "hbm", "avg_us": round(avg_us[s2t.PROF_DYE], 2),
+                         "bytes": dye_bytes, "achieved": round(dye_gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(dye_gbs / HBM_PEAK_GBS, 4)},
+            "kernels_us": {"k_rows": round(avg_us[s2t.PROF_ROWS], 2),
+                           "k_cols": round(avg_us[s2t.PROF_COLS], 2),
+                           "dye (3 launches)": round(avg_us[s2t.PROF_DYE], 2)}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,9 +304,11 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", choices=["step", "ensemble", "serve"], default="step",
+    ap.add_argument("--workload", choices=["step", "ensemble", "serve", "decaying"],
+                    default="step",
                     help="step: one 4096^2 instance (default); ensemble: 64 mixed-size members; "
-                         "serve: Poisson arrivals into the serving loop")
+                         "serve: Poisson arrivals into the serving loop; decaying: 1024^2 "
+                         "decaying turbulence with dye on the 2048^2 grid")
     ap.add_argument("--plain-steps", type=int, default=50,
                     help="ensemble: steps per member of the plain-loop baseline")
     ap.add_argument("--rates", type=str, default="",
@@ -264,6 +330,9 @@ def main(argv=None):
         return 0
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(s2t, args.steps, args.warmup, args.plain_steps)))
+        return 0
+    if args.workload == "decaying":
+        print(json.dumps(bench_decaying(s2t, args.steps, args.warmup)))
         return 0
     if args.workload == "serve":
         rates = [float(r) for r in args.rates.split(",")] if args.rates else [5.0, 15.0, 30.0]

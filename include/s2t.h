@@ -330,7 +330,8 @@ int s2t_profile_sampled(s2t_solver* h, unsigned mask, int every);
  * Waits for queued work. */
 int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);
 /* Algorithmic HBM bytes of one launch of a kernel kind for this solver's size and the current
- * options (stage 0 of a step when the kinds differ by stage). */
+ * options (stage 0 of a step when the kinds differ by stage); for S2T_PROF_DYE, of one dye step
+ * (its three launches, 56 B per dye texel + 24 B per grid point). */
 int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes);
 /* Measurement helper with no counterpart in the reference: `reps` launches of a plain 16-byte
  * streaming copy kernel (k_copy) of `bytes` bytes on a private stream; average ms per copy. Its

@@ -475,6 +475,12 @@ int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes) {
     // read P', Q', w0 (+ wcur), write wout and omega', u', v'
     b = 2 * plane + st * (stage == 0 ? 2 : 3) + 3 * plane;
     if (force) b += stage == 0 ? plane + st : st;
+  } else if (kind == S2T_PROF_DYE) {
+    // one BFECC dye step, its three launches together: forward and backward advection each read
+    // and write one rgba16f texture (8 B per texel); the correction reads phi0 and phi_hat and
+    // writes the result; each launch reads the N x N float2 velocity once
+    const double m = s->m;
+    b = 7.0 * m * m * 8.0 + 3.0 * n * n * 8.0;
   } else {
     return api_bad("no byte model for this kernel kind");
   }

@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "aux.hpp"
+#include "core.hpp"
 #include "physics.hpp"
 
 namespace s2t {
@@ -348,8 +349,10 @@ void diagnostics(hipStream_t stm, const float2* st, const double2* wtab, double*
 // dye: BFECC with MacCormack limiter (src/shaders/dye.wgsl:1-113) on binary16 rgba texels with
 // FP32 bilinear filtering and repeat addressing
 // ---------------------------------------------------------------------------------------------
+// Repeat addressing by masks: the grid and dye sizes are powers of two (dye_step checks), and
+// i & (m - 1) is the non-negative residue of i mod m for negative i too (two's complement).
 __device__ __forceinline__ float4 texel(const ushort4* __restrict__ tex, int m, int ix, int iy) {
-  const int x = ((ix % m) + m) % m, y = ((iy % m) + m) % m;
+  const int x = ix & (m - 1), y = iy & (m - 1);
   const ushort4 v = tex[(size_t)y * m + x];
   return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
                      __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
@@ -369,7 +372,7 @@ __device__ __forceinline__ float4 sample_tex(const ushort4* __restrict__ tex, in
   return lerp4(top, bot, fy);
 }
 __device__ __forceinline__ float2 vel_at(const float2* __restrict__ vel, int n, int ix, int iy) {
-  const int x = ((ix % n) + n) % n, y = ((iy % n) + n) % n;
+  const int x = ix & (n - 1), y = iy & (n - 1);
   return vel[(size_t)y * n + x];
 }
 __device__ __forceinline__ float2 sample_vel(const float2* __restrict__ vel, int n, float px,
@@ -461,6 +464,8 @@ __global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __res
 
 void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
               const DevState* dev, double dt, float fade, const Splats& sp, int n, int m) {
+  if (n <= 0 || m <= 0 || (n & (n - 1)) != 0 || (m & (m - 1)) != 0)
+    throw Fail(S2T_EINVAL, "dye step: grid and dye sizes must be powers of two");
   const dim3 blk(16, 16), grd((m + 15) / 16, (m + 15) / 16);
   const int p = parity;
   // bind-group parity of src/gpu/solver.js:299-303

@@ -405,11 +405,23 @@ __device__ __forceinline__ double dye_dt(const DevState* dev, double dt) {
   return dev ? dev->dt : dt;
 }
 
+// Texel of this thread. Consecutive workgroups go round-robin over the 8 XCDs, so each XCD is
+// given a contiguous band of block rows (when the block count divides by 8): the bilinear gathers
+// of neighbouring blocks read the same cache lines, which then stay in one L2.
+__device__ __forceinline__ int2 dye_texel() {
+  const int gx = (int)gridDim.x, total = gx * (int)gridDim.y;
+  int b = (int)blockIdx.y * gx + (int)blockIdx.x;
+  if (total % 8 == 0) b = (b % 8) * (total / 8) + b / 8;
+  return make_int2((b % gx) * (int)blockDim.x + (int)threadIdx.x,
+                   (b / gx) * (int)blockDim.y + (int)threadIdx.y);
+}
+
 // advect_fwd (sign +1) and advect_back (sign -1), dye.wgsl:58-74
 __global__ void k_dye_advect(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
                              const float2* __restrict__ vel, const DevState* dev, double dt,
                              float sign, int n, int m) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int2 xy = dye_texel();
+  const int x = xy.x, y = xy.y;
   if (x >= m || y >= m) return;
   const float h = sign * (float)dye_dt(dev, dt) / TWO_PI_F;
   const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
@@ -421,7 +433,8 @@ __global__ void k_dye_advect(const ushort4* __restrict__ src, ushort4* __restric
 __global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __restrict__ aux,
                            ushort4* __restrict__ dst, const float2* __restrict__ vel,
                            const DevState* dev, double dt, float fade, Splats sp, int n, int m) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int2 xy = dye_texel();
+  const int x = xy.x, y = xy.y;
   if (x >= m || y >= m) return;
   const float h = (float)dye_dt(dev, dt) / TWO_PI_F;
   const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;

@@ -34,6 +34,41 @@ def hashf(a, b):
     return h.astype(np.float32) * np.float32(2.3283064365386963e-10)
 
 
+def canonical_half(kx, ky):
+    """The half of the modes that carries its own random phase: ky > 0, or ky = 0 and kx > 0
+    (solver.wgsl:80-86 for the forcing, src/gpu/solver.js:401-405 for fillSpectrum); every other
+    mode takes the conjugate of its mirror's value.
+
+    Pinned by the known answers of FORCE_KAT / FILL_KAT (tests/test_oracle_pins.py), which include
+    modes with ky > 0 and kx < 0 (canonical here, not under a kx-first rule) and modes with
+    ky = 0.
+    """
+    return (ky > 0) | ((ky == 0) & (kx > 0))
+
+
+def force_phase(idx, step_index):
+    """Forcing phase of canonical index idx = row*N + col: TWO_PI * hashf(idx, stepIndex), the
+    product in FP32 as in the WGSL (solver.wgsl:87).
+
+    Pinned by known answers computed in C (tests/golden/hash_kat.c), by the zero means of
+    e^{i phase} and e^{2 i phase} over the annulus and by zero step-to-step correlation
+    (tests/test_oracle_pins.py).
+    """
+    return (F32_TWO_PI * hashf(np.asarray(idx, np.uint32), U32(step_index))).astype(np.float64)
+
+
+def fill_phase(idx, seed):
+    """fillSpectrum phase of canonical index idx = row*n + col:
+    2 pi hash32(idx ^ hash32(seed)) / 2^32 in FP64 (src/gpu/solver.js:387, 404).
+
+    Pinned by known answers computed in C (tests/golden/hash_kat.c) and by the zero means of
+    e^{i phase} and e^{2 i phase} (tests/test_oracle_pins.py).
+    """
+    salt = hash1(np.uint32(int(seed) & 0xFFFFFFFF))
+    h = hash1(np.asarray(idx, np.uint32) ^ salt).astype(np.float64)
+    return TWO_PI * h / 4294967296.0
+
+
 # ---------------------------------------------------------------------------------------------
 # parameters
 # ---------------------------------------------------------------------------------------------
@@ -72,8 +107,12 @@ def stochastic_forcing(n, amp, k_f, width, step_index, dt):
     Hermitian. phase = 2 pi hashf(row*N + col, stepIndex); magnitude
     amp * env * FORCE_NORM / sqrt(max(dt, 1e-6)).
 
-    Pinned by: Hermitian symmetry, annulus support, 1/sqrt(dt) scaling and the closed-form energy
-    injection rate of ``injection_rate`` (tests/test_oracle_semantics.py).
+    Pinned by: known answers of the phase composition for modes of both halves and of the ky = 0
+    row, computed in C (``force_phase``, ``canonical_half``); zero means of e^{i phase} and
+    e^{2 i phase} over the annulus and zero lag-1 / lag-2 step correlation (a pi u phase, a
+    hash1(idx ^ step) or a hash-then-xor composition, each planted in this oracle, fails one of
+    them: tests/test_oracle_pins.py); Hermitian symmetry, annulus support, 1/sqrt(dt) scaling and
+    the closed-form energy injection rate of ``injection_rate`` (tests/test_oracle_semantics.py).
     """
     g = Grid(n)
     out = np.zeros((n, n), np.complex128)
@@ -83,12 +122,11 @@ def stochastic_forcing(n, amp, k_f, width, step_index, dt):
     rows, cols = np.nonzero(env > 1e-4)
     kx = g.kx[rows, cols]
     ky = g.ky[rows, cols]
-    canon = (ky > 0) | ((ky == 0) & (kx > 0))
+    canon = canonical_half(kx, ky)
     hr = np.where(canon, rows, (n - rows) % n)
     hc = np.where(canon, cols, (n - cols) % n)
     sgn = np.where(canon, 1.0, -1.0)
-    ph = F32_TWO_PI.astype(np.float64) * hashf((hr * n + hc).astype(np.uint32),
-                                                 U32(step_index)).astype(np.float64)
+    ph = force_phase((hr * n + hc).astype(np.uint32), step_index)
     mag = amp * env[rows, cols] * force_norm(n) / np.sqrt(max(dt, 1e-6))
     out[rows, cols] = mag * (np.cos(ph) + 1j * sgn * np.sin(ph))
     return out
@@ -151,6 +189,13 @@ def pointer_force_spectrum(n, splats):
 # ---------------------------------------------------------------------------------------------
 # seeding and scenarios
 # ---------------------------------------------------------------------------------------------
+def fill_amplitude(k2, k0):
+    """Mode magnitude sqrt(k^2 / (1 + (|k|/k0)^8)) of fillSpectrum before the rms scale
+    (src/gpu/solver.js:398-400). Pinned by magnitude ratios between modes
+    (tests/test_oracle_pins.py)."""
+    return np.sqrt(k2 / (1.0 + (np.sqrt(k2) / k0) ** 8))
+
+
 def fill_spectrum(n, k0=14.0, amplitude=6.0, seed=1):
     """Seeded band-limited spectrum of src/gpu/solver.js:384-418, returned as complex64 N x N.
 
@@ -159,22 +204,21 @@ def fill_spectrum(n, k0=14.0, amplitude=6.0, seed=1):
     mirror takes the conjugate), stored as float32, then every value is rescaled (FP64 product,
     float32 store) so that sqrt(sum amp^2)/N^2 = amplitude.
 
-    Pinned by: Hermitian symmetry, band support, the rms normalisation (Parseval: the physical
-    field has rms = amplitude) and hash known answers.
+    Pinned by: known answers of the phase for modes of both halves and of the ky = 0 row,
+    computed in C (``fill_phase``), and magnitude ratios sqrt(env(k1)/env(k2)) between modes
+    (a salt of seed instead of hash32(seed), a hash-then-xor salt, or env instead of sqrt(env),
+    each planted in this oracle, fails them: tests/test_oracle_pins.py); Hermitian symmetry, band
+    support and the rms normalisation (Parseval: the physical field has rms = amplitude).
     """
     g = Grid(n)
     ok = g.band & (g.k2 > 0)
-    kr = np.sqrt(g.k2)
-    env = g.k2 / (1.0 + (kr / k0) ** 8)
-    amp = np.where(ok, np.sqrt(env), 0.0)
+    amp = np.where(ok, fill_amplitude(g.k2, k0), 0.0)
     rows, cols = np.indices((n, n))
-    canon = (g.ky > 0) | ((g.ky == 0) & (g.kx > 0))
+    canon = canonical_half(g.kx, g.ky)
     hr = np.where(canon, rows, (n - rows) % n)
     hc = np.where(canon, cols, (n - cols) % n)
     sgn = np.where(canon, 1.0, -1.0)
-    salt = hash1(np.uint32(seed & 0xFFFFFFFF))
-    h = hash1((hr * n + hc).astype(np.uint32) ^ salt).astype(np.float64)
-    phase = TWO_PI * h / 4294967296.0
+    phase = fill_phase((hr * n + hc).astype(np.uint32), seed)
     re = np.where(ok, amp * np.cos(phase), 0.0).astype(np.float32)
     im = np.where(ok, sgn * amp * np.sin(phase), 0.0).astype(np.float32)
     total = float(np.sum(amp[ok] ** 2))
@@ -249,7 +293,15 @@ def blobs(n, items):
 
 
 def scenario_field(name, n, cfg=None):
-    """Physical initial vorticity of the reference scenarios (src/scenarios.js:57-165)."""
+    """Physical initial vorticity of the reference scenarios (src/scenarios.js:57-165).
+
+    Pinned by what the mathematics fixes (tests/test_oracle_pins.py): Taylor-Green has exactly
+    the four modes (+-2, +-2) of 6/4 N^2; the dipole's moments (circulation 12 per core, the
+    separation, the sign, the column, the width; a sign swap, a doubled offset and a column
+    misread, planted here, fail them); the merger's circulation, centroid and second moments;
+    each polygon core's circulation and centroid on its jittered radius; the shear layer's
+    velocity jump, width and perturbation amplitude.
+    """
     cfg = cfg or {}
     if name == "taylor-green":
         return _grid(n, lambda x, y: 6.0 * np.cos(2 * x) * np.cos(2 * y))
@@ -311,7 +363,13 @@ def to_half_truncating(value):
 
 
 def paint_bands(m):
-    """'bands' dye pattern of solver.js:446-473 as an (m, m, 4) float16 array."""
+    """'bands' dye pattern of solver.js:446-473 as an (m, m, 4) float16 array.
+
+    Pinned by landmarks the formula fixes (tests/test_oracle_pins.py): the band level 1/2 on the
+    zeros of sin(12 pi fy), bright and dark band centres, pure warm and cool colours at
+    fx = 1/4 and 3/4, the 0.9 factor, truncation to binary16, alpha 1 and six bands; x / y or
+    warm / cool swapped, planted in its output, fail them.
+    """
     fy = np.arange(m) / m
     fx = np.arange(m) / m
     band = 0.5 + 0.5 * np.tanh(6 * np.sin(fy * np.pi * 12))

@@ -112,6 +112,18 @@ int s2t_set_splats(s2t_solver* h, const s2t_splat* splats, int count);
 /* Solver.seed({k0, amplitude, seed}) via fillSpectrum (solver.js:347-359, 384-418); amplitude 0
  * gives a fluid at rest. Resets step index, time and dt like Solver.reset (solver.js:420-444). */
 int s2t_seed_spectrum(s2t_solver* h, double k0, double amplitude, uint32_t seed);
+/* fillSpectrum(data, k0, amplitude, seed) into caller memory with no allocation. No counterpart
+ * in the reference (its fillSpectrum, src/gpu/solver.js:384-418, fills a host array); for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc. Writes the spectrum of s2t_seed_spectrum, bit for bit, as 2*n*n floats
+ * in the fixture layout (both halves, zero out of band) to spectrum_dev, device memory of the
+ * current device; enqueued on `stream` (a hipStream_t; NULL = the null stream), no wait. Calls no
+ * hipMalloc, hipFree or allocating library routine: the per-column sums live in a fixed device
+ * array of the library (one per GPU, about 11 KB, held until the process ends), so n is at most
+ * 4096. Calls that overlap on one GPU share that array and race: order them (one stream, or
+ * events). */
+int s2t_fill_spectrum_noalloc(int n, double k0, double amplitude, uint32_t seed,
+                              float* spectrum_dev, void* stream);
 /* Solver.seedPhysical(values) (solver.js:361-382): n*n physical vorticity -> forward FFT ->
  * 2/3 mask, k = 0 zeroed (src/shaders/seed.wgsl:4-17). Also resets like Solver.reset. */
 int s2t_seed_physical(s2t_solver* h, const float* field);

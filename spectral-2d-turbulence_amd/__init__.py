@@ -45,7 +45,7 @@ EXPORTS = [
     "s2t_batch_sync", "s2t_batch_autotune", "s2t_batch_column_width",
     "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
     "s2t_server_read_state", "s2t_server_cancel", "s2t_server_set_callback", "s2t_server_drain",
-    "s2t_server_get_stats",
+    "s2t_server_get_stats", "s2t_fill_spectrum_noalloc",
 ]
 SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4, 5
@@ -173,6 +173,7 @@ def load(path=None):
         "s2t_server_set_callback": (ip, [vp, SERVER_CALLBACK, vp]),
         "s2t_server_drain": (ip, [vp, dp]),
         "s2t_server_get_stats": (ip, [vp, ctypes.POINTER(ServerStats)]),
+        "s2t_fill_spectrum_noalloc": (ip, [ip, dp, dp, ctypes.c_uint32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

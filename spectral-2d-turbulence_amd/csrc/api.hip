@@ -119,6 +119,21 @@ int s2t_seed_spectrum(s2t_solver* h, double k0, double amplitude, uint32_t seed)
   S2T_API_END
 }
 
+int s2t_fill_spectrum_noalloc(int n, double k0, double amplitude, uint32_t seed,
+                              float* spectrum_dev, void* stream) {
+  const std::string e = size_error(n);
+  if (n > noalloc_max_n())
+    return api_bad("s2t_fill_spectrum_noalloc: grid size " + std::to_string(n) +
+                   " exceeds its fixed scratch (at most " + std::to_string(noalloc_max_n()) + ")");
+  if (!e.empty()) return api_bad(e);
+  if (!(k0 > 0)) return api_bad("k0 must be positive");
+  if (!spectrum_dev) return api_bad("s2t_fill_spectrum_noalloc: spectrum_dev is NULL");
+  S2T_API_BEGIN
+  fill_spectrum_noalloc(static_cast<hipStream_t>(stream), reinterpret_cast<float2*>(spectrum_dev),
+                        n, k0, amplitude, seed);
+  S2T_API_END
+}
+
 int s2t_seed_physical(s2t_solver* h, const float* field) {
   SOLVER(h);
   if (!field) return api_bad("field is NULL");

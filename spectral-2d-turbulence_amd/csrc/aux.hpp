@@ -14,8 +14,13 @@ void vpack_full(hipStream_t st, const float2* w, float2* pa, float2* pb, int n);
 void assemble_full(hipStream_t st, const float2* z, const float2* fhat, float2* rhs,
                    const Phys& ph, uint32_t step, double dt, int n);
 void real_to_complex(hipStream_t st, const float* r, float2* c, size_t n);
-void seed_spectrum(hipStream_t st, float2* state, double* scratch, int n, double k0,
+void seed_spectrum(hipStream_t st, float2* state, double* sums, int n, double k0,
                    double amplitude, uint32_t seed);
+// fillSpectrum into a full n x n device array with no allocation (its scratch is a fixed device
+// array of the library); n up to noalloc_max_n()
+void fill_spectrum_noalloc(hipStream_t st, float2* full, int n, double k0, double amplitude,
+                           uint32_t seed);
+int noalloc_max_n();
 void diagnostics(hipStream_t st, const float2* state, const double2* wtab, double* bins,
                  double* zonal, int n);
 void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "aux.hpp"
 #include "core.hpp"
@@ -182,6 +184,28 @@ __global__ void k_seed_scale(RGeo g, double amplitude, double* __restrict__ cols
   }
 }
 
+// fillSpectrum value of the in-band mode (kx, ky) != 0 at full-grid position (row, col): magnitude
+// sqrt(k^2 / (1 + (|k|/k0)^8)), phase 2 pi hash32(canonicalIdx ^ salt) / 2^32 (the mirror of a
+// non-canonical mode gives the index, with the sine negated), stored as f32 and then rescaled by
+// the FP64 factor sc (solver.js:396-417). A mode and its mirror come out as exact conjugates.
+__device__ __forceinline__ float2 seed_mode(int n, int kx, int ky, int row, int col, double k0,
+                                            uint32_t salt, double sc) {
+  const double amp = seed_amp(kx, ky, k0);
+  const bool canon = ky > 0 || (ky == 0 && kx > 0);
+  const int hr = canon ? row : (n - row) % n;
+  const int hc = canon ? col : (n - col) % n;
+  const double sgn = canon ? 1.0 : -1.0;
+  const double phase = 2.0 * M_PI * (double)hash1((uint32_t)(hr * n + hc) ^ salt) / 4294967296.0;
+  float re = (float)(amp * cos(phase));
+  float im = (float)(sgn * amp * sin(phase));
+  if (sc > 0.0) {
+    re = (float)((double)re * sc);
+    im = (float)((double)im * sc);
+  }
+  return make_float2(re, im);
+}
+
+// the solver's compact half-spectrum layout
 __global__ void k_seed_fill(RGeo g, double k0, uint32_t salt, const double* __restrict__ scale,
                             float2* __restrict__ st) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,41 +213,92 @@ __global__ void k_seed_fill(RGeo g, double k0, uint32_t salt, const double* __re
   const int kx = (int)(i / g.pitch), j = (int)(i % g.pitch);
   float2 v = make_float2(0.f, 0.f);
   const int ky = j <= g.K ? j : j - g.NJ;
-  if (j < g.NJ && !(kx == 0 && ky == 0)) {
-    const int n = g.n;
-    const int row = (ky + n) % n, col = kx;
-    const double amp = seed_amp(kx, ky, k0);
-    const bool canon = ky > 0 || (ky == 0 && kx > 0);
-    const int hr = canon ? row : (n - row) % n;
-    const int hc = canon ? col : (n - col) % n;
-    const double sgn = canon ? 1.0 : -1.0;
-    const double phase = 2.0 * M_PI * (double)hash1((uint32_t)(hr * n + hc) ^ salt) / 4294967296.0;
-    float re = (float)(amp * cos(phase));
-    float im = (float)(sgn * amp * sin(phase));
-    const double sc = *scale;
-    if (sc > 0.0) {
-      re = (float)((double)re * sc);
-      im = (float)((double)im * sc);
-    }
-    v = make_float2(re, im);
-  }
+  if (j < g.NJ && !(kx == 0 && ky == 0))
+    v = seed_mode(g.n, kx, ky, (ky + g.n) % g.n, kx, k0, salt, *scale);
   st[i] = v;
 }
 
-void seed_spectrum(hipStream_t stm, float2* st, double* scratch, int n, double k0, double amplitude,
-                   uint32_t seed) {
+// the full N x N fixture layout (both halves, zero out of band)
+__global__ void k_seed_fill_full(RGeo g, double k0, uint32_t salt, const double* __restrict__ scale,
+                                 float2* __restrict__ full) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)g.n * g.n) return;
+  const int row = (int)(i / g.n), col = (int)(i % g.n);
+  const int kx = wn(col, g.n), ky = wn(row, g.n);
+  float2 v = make_float2(0.f, 0.f);
+  if (abs(kx) <= g.K && abs(ky) <= g.K && !(kx == 0 && ky == 0))
+    v = seed_mode(g.n, kx, ky, row, col, k0, salt, *scale);
+  full[i] = v;
+}
+
+void seed_spectrum(hipStream_t stm, float2* st, double* scratch_sums, int n, double k0,
+                   double amplitude, uint32_t seed) {
   const RGeo g = rgeo(n);
   if (amplitude > 0.0) {
-    hipLaunchKernelGGL(k_seed_colsum, dim3(blocks(g.KC, 64)), dim3(64), 0, stm, g, k0, scratch);
+    hipLaunchKernelGGL(k_seed_colsum, dim3(blocks(g.KC, 64)), dim3(64), 0, stm, g, k0,
+                       scratch_sums);
     check_launch("k_seed_colsum");
-    hipLaunchKernelGGL(k_seed_scale, dim3(1), dim3(256), 0, stm, g, amplitude, scratch);
+    hipLaunchKernelGGL(k_seed_scale, dim3(1), dim3(256), 0, stm, g, amplitude, scratch_sums);
     check_launch("k_seed_scale");
     hipLaunchKernelGGL(k_seed_fill, dim3(blocks((size_t)g.KC * g.pitch, 256)), dim3(256), 0, stm,
-                       g, k0, hash1(seed), scratch + g.KC, st);
+                       g, k0, hash1(seed), scratch_sums + g.KC, st);
     check_launch("k_seed_fill");
   } else {
     fill_u32(stm, st, (size_t)g.KC * g.pitch * 2, 0u);
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// fillSpectrum into caller memory with no allocation (s2t_fill_spectrum_noalloc): the column
+// sums and the scale live in this fixed device array, one copy per GPU, held until the process
+// ends. Host code never passes the array's name (a host-side stand-in) to a kernel: it uses the
+// device address that hipGetSymbolAddress gives on each device.
+// ---------------------------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_N = 4096;
+constexpr int NOALLOC_DOUBLES = NOALLOC_MAX_N / 3 + 3;  // KC column sums + the scale, at most
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device, looked up on the first call there
+static double* scratch_on_device() {
+  int dev = 0;
+  hipchk(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    throw Fail(S2T_EINVAL, "device index " + std::to_string(dev) + " beyond the no-alloc table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hipchk(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+int noalloc_max_n() { return NOALLOC_MAX_N; }
+
+void fill_spectrum_noalloc(hipStream_t stm, float2* full, int n, double k0, double amplitude,
+                           uint32_t seed) {
+  if (n > NOALLOC_MAX_N)
+    throw Fail(S2T_EINVAL, "s2t_fill_spectrum_noalloc: grid size " + std::to_string(n) +
+                               " exceeds its fixed scratch (at most " +
+                               std::to_string(NOALLOC_MAX_N) + ")");
+  const RGeo g = rgeo(n);
+  if (!(amplitude > 0.0)) {
+    fill_u32(stm, full, (size_t)n * n * 2, 0u);
+    return;
+  }
+  double* sums = scratch_on_device();
+  // every column sum and the scale are written before they are read: nothing to clear
+  hipLaunchKernelGGL(k_seed_colsum, dim3(blocks(g.KC, 64)), dim3(64), 0, stm, g, k0, sums);
+  check_launch("k_seed_colsum");
+  hipLaunchKernelGGL(k_seed_scale, dim3(1), dim3(256), 0, stm, g, amplitude, sums);
+  check_launch("k_seed_scale");
+  hipLaunchKernelGGL(k_seed_fill_full, dim3(blocks((size_t)n * n, 256)), dim3(256), 0, stm, g, k0,
+                     hash1(seed), sums + g.KC, full);
+  check_launch("k_seed_fill_full");
 }
 
 // ---------------------------------------------------------------------------------------------

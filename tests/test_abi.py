@@ -106,3 +106,12 @@ def test_argument_checks_before_any_device_work(s2t):
     assert lib.s2t_autotune(None, 3, None, None, 0) == s2t.S2T_EINVAL
     assert lib.s2t_set_clock(None, 0.0, 0) == s2t.S2T_EINVAL
     assert lib.s2t_profile_sampled(None, 1, 4) == s2t.S2T_EINVAL
+    # the no-allocation fill: sizes beyond its fixed scratch, bad sizes, bad k0, NULL output
+    fake = ctypes.c_void_p(4096)  # never dereferenced: every call below fails its checks first
+    assert lib.s2t_fill_spectrum_noalloc(8192, 8.0, 1.0, 1, fake, None) == s2t.S2T_EINVAL
+    assert "exceeds its fixed scratch (at most 4096)" in lib.s2t_last_error().decode()
+    assert lib.s2t_fill_spectrum_noalloc(96, 8.0, 1.0, 1, fake, None) == s2t.S2T_EINVAL
+    assert "4^a" in lib.s2t_last_error().decode()
+    assert lib.s2t_fill_spectrum_noalloc(256, 0.0, 1.0, 1, fake, None) == s2t.S2T_EINVAL
+    assert lib.s2t_fill_spectrum_noalloc(256, 8.0, 1.0, 1, None, None) == s2t.S2T_EINVAL
+    assert "NULL" in lib.s2t_last_error().decode()

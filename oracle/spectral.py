@@ -22,6 +22,58 @@ import numpy as np
 
 TWO_PI = 2.0 * np.pi
 
+_WORKERS = 1
+_POOL = None
+
+
+def set_fft_workers(k):
+    """Threads of the scheme: 1 (the default) runs everything on one thread with numpy.fft, the
+    CPU baseline's setting; k > 1 runs the 2-D transforms with scipy.fft on k workers and the
+    pointwise stages over k row bands on a thread pool (NumPy releases the GIL inside its
+    loops), which the tests use to run long FP64 comparisons in reasonable time. The arithmetic
+    of every element is the same either way; the transforms are pocketfft in both libraries and
+    agree to FP64 rounding."""
+    global _WORKERS, _POOL
+    _WORKERS = max(1, int(k))
+    if _POOL is not None:
+        _POOL.shutdown()
+        _POOL = None
+
+
+def _rows(fn, n, count=1):
+    """fn(r0, r1) -> `count` complex arrays of rows [r0, r1). On one thread fn runs once over all
+    rows (the plain NumPy expressions); with k workers it runs over k row bands on a thread pool
+    and the bands are written into complex128 outputs of n rows."""
+    global _POOL
+    if _WORKERS == 1 or n < 512:  # small grids: thread overhead exceeds the work
+        return fn(0, n)
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=_WORKERS)
+    outs = [np.empty((n, n), np.complex128) for _ in range(count)]
+
+    def band(b):
+        res = fn(b[0], b[1])
+        for o, r in zip(outs, res if count > 1 else (res,)):
+            o[b[0]:b[1]] = r
+    edges = np.linspace(0, n, _WORKERS + 1).astype(int)
+    list(_POOL.map(band, zip(edges[:-1], edges[1:])))
+    return tuple(outs) if count > 1 else outs[0]
+
+
+def fft2(a):
+    if _WORKERS > 1 and a.shape[0] >= 512:
+        import scipy.fft
+        return scipy.fft.fft2(a, workers=_WORKERS)
+    return np.fft.fft2(a)
+
+
+def ifft2(a):
+    if _WORKERS > 1 and a.shape[0] >= 512:
+        import scipy.fft
+        return scipy.fft.ifft2(a, workers=_WORKERS)
+    return np.fft.ifft2(a)
+
 
 class Grid:
     """Wavenumbers, dealias mask and inverse Laplacian for an N x N periodic grid.
@@ -75,27 +127,31 @@ class Scheme:
     # -- state ---------------------------------------------------------------------------------
     def set_physical(self, field):
         """Transform a real field, mask it to the band and zero the mean (spectral2d.py:33-35)."""
-        spec = np.fft.fft2(np.asarray(field, np.float64))
+        spec = fft2(np.asarray(field, np.float64))
         spec = np.where(self.g.band, spec, 0.0)
         spec[0, 0] = 0.0
         self.w = spec.astype(np.complex128)
 
     def physical(self, w=None):
-        return np.fft.ifft2(self.w if w is None else w).real
+        return ifft2(self.w if w is None else w).real
 
     # -- right-hand side -----------------------------------------------------------------------
     def packed_spectra(self, w):
         """A = w_hat + i u_hat and B = v_hat (solver.wgsl velocity_pack; spectral2d.py:45-50)."""
         g = self.g
-        uh = 1j * g.ky * g.inv_k2 * w
-        vh = -1j * g.kx * g.inv_k2 * w
-        return w + 1j * uh, vh
+
+        def band(r0, r1):
+            ws = w[r0:r1]
+            uh = 1j * g.ky[r0:r1] * g.inv_k2[r0:r1] * ws
+            vh = -1j * g.kx[r0:r1] * g.inv_k2[r0:r1] * ws
+            return ws + 1j * uh, vh
+        return _rows(band, self.n, 2)
 
     def fields(self, w):
         """Physical omega, u, v from one packed inverse transform plus one for v."""
         a, b = self.packed_spectra(w)
-        pa = np.fft.ifft2(a)
-        pb = np.fft.ifft2(b)
+        pa = ifft2(a)
+        pb = ifft2(b)
         return pa.real, pa.imag, pb.real
 
     def velocity(self, w=None):
@@ -105,19 +161,31 @@ class Scheme:
     def unpack(self, z):
         """Hermitian unpack of a transform of two real fields (solver.wgsl assemble:152-155)."""
         m = self.g.mirror
-        zm = np.conj(z[np.ix_(m, m)])
-        return 0.5 * (z + zm), -0.5j * (z - zm)
+
+        def band(r0, r1):
+            zs = z[r0:r1]
+            zm = np.conj(z[np.ix_(m[r0:r1], m)])
+            return 0.5 * (zs + zm), -0.5j * (zs - zm)
+        return _rows(band, self.n, 2)
 
     def product_transform(self, w):
         """Z_hat = FFT(u w + i v w): the one forward transform of the packed product."""
         omega, u, v = self.fields(w)
-        return np.fft.fft2(u * omega + 1j * (v * omega))
+
+        def band(r0, r1):
+            om = omega[r0:r1]
+            return u[r0:r1] * om + 1j * (v[r0:r1] * om)
+        return fft2(_rows(band, self.n))
 
     def nhat_from_z(self, z):
         """N_hat = (i kx q1 + i ky q2) masked, from the packed product spectrum."""
         q1, q2 = self.unpack(z)
         g = self.g
-        return np.where(g.band, 1j * g.kx * q1 + 1j * g.ky * q2, 0.0)
+
+        def band(r0, r1):
+            return np.where(g.band[r0:r1], 1j * g.kx[r0:r1] * q1[r0:r1]
+                            + 1j * g.ky[r0:r1] * q2[r0:r1], 0.0)
+        return _rows(band, self.n)
 
     def nonlinear(self, w):
         return self.nhat_from_z(self.product_transform(w))
@@ -126,7 +194,7 @@ class Scheme:
         """Unpacked evaluation with two separate forward transforms (spectral2d.py:139)."""
         omega, u, v = self.fields(w)
         g = self.g
-        nh = 1j * g.kx * np.fft.fft2(u * omega) + 1j * g.ky * np.fft.fft2(v * omega)
+        nh = 1j * g.kx * fft2(u * omega) + 1j * g.ky * fft2(v * omega)
         return np.where(g.band, nh, 0.0)
 
     def rhs(self, w, forcing=None):
@@ -145,7 +213,15 @@ class Scheme:
         return min(self.dt_max, self.cfl * self.g.dx / speed)
 
     def ifactor(self, s):
-        return np.exp(self.lin * s)
+        return _rows(lambda r0, r1: np.exp(self.lin[r0:r1] * s), self.n)
+
+    def _combine(self, ca, ea, w0, cb, eb, wc, dt, f):
+        """ca ea w0 + cb eb (wc + dt f) over row bands (ca = 0: the first stage)."""
+
+        def band(r0, r1):
+            x = cb * eb[r0:r1] * (wc[r0:r1] + dt * f[r0:r1])
+            return x if ca == 0.0 else ca * ea[r0:r1] * w0[r0:r1] + x
+        return _rows(band, self.n)
 
     def step(self, dt=None, forcing=None):
         """One IF-SSP-RK3 step; k = 0 is zeroed at the end (spectral2d.py:76-90).
@@ -158,11 +234,11 @@ class Scheme:
         if dt is None:
             dt = self.cfl_dt()
         w0 = self.w
-        w1 = self.ifactor(dt) * (w0 + dt * self.rhs(w0, forcing))
-        w2 = 0.75 * self.ifactor(0.5 * dt) * w0 + 0.25 * self.ifactor(-0.5 * dt) * (
-            w1 + dt * self.rhs(w1, forcing))
-        w3 = (1.0 / 3.0) * self.ifactor(dt) * w0 + (2.0 / 3.0) * self.ifactor(0.5 * dt) * (
-            w2 + dt * self.rhs(w2, forcing))
+        w1 = self._combine(0.0, None, w0, 1.0, self.ifactor(dt), w0, dt, self.rhs(w0, forcing))
+        w2 = self._combine(0.75, self.ifactor(0.5 * dt), w0, 0.25, self.ifactor(-0.5 * dt), w1, dt,
+                           self.rhs(w1, forcing))
+        w3 = self._combine(1.0 / 3.0, self.ifactor(dt), w0, 2.0 / 3.0, self.ifactor(0.5 * dt), w2,
+                           dt, self.rhs(w2, forcing))
         w3[0, 0] = 0.0
         self.w = w3
         self.t += dt

@@ -13,10 +13,15 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# the FP64 oracle runs its transforms and pointwise stages on the host cores (at most 16, one
+# GPU's share of a box)
+from oracle import spectral as _spectral  # noqa: E402
+
+_spectral.set_fft_workers(min(16, len(os.sched_getaffinity(0))))
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
-    config.addinivalue_line("markers", "slow: long CPU oracle runs (set S2T_SLOW=1)")
 
 
 def load_binding():

@@ -37,7 +37,10 @@ def close_writes(asm, window=2):
     """(kernel, store, writer) for VALU writes of a wide store's data within `window`
     instructions after it."""
     found = []
+    funcs = set(re.findall(r"\.type\s+(_Z\S+?),\s*@function", asm))
     for name in re.findall(r"^(_Z\S+):", asm, re.M):
+        if funcs and name not in funcs:
+            continue  # a data object (the device arrays), not a kernel
         i = asm.index(name + ":")
         body = [l.strip() for l in asm[i:asm.index(".Lfunc_end", i)].splitlines()]
         body = [l for l in body if l and not l.startswith((";", "."))]

@@ -87,8 +87,7 @@ def test_diagnostics_parseval():
     assert s.zonal_mean_u() == pytest.approx(u.mean(axis=1), abs=1e-13)
 
 
-@pytest.mark.slow
-@pytest.mark.skipif(not os.environ.get("S2T_SLOW"), reason="long oracle run; set S2T_SLOW=1")
+@pytest.mark.skipif(not os.environ.get("S2T_SLOW"), reason="long oracle run (about a minute); set S2T_SLOW=1")
 def test_decaying_slope():
     s = Scheme(256, nu=nu_from_rate(0.5, 256, 4), p=4, cfl=0.4, dt_max=0.05)
     s.set_physical(random_field(256, k0=24, seed=5, amplitude=6.0))

@@ -296,6 +296,138 @@ def bench_decaying(s2t, steps, warmup):
                            "dye (3 launches)": round(avg_us[s2t.PROF_DYE], 2)}}
 
 
+def dissipation_rate(spectrum, n, params):
+    """Energy removed per unit time by drag and hyperviscosity, from the FP64 E(k) bins:
+    sum_b 2 (mu + nu_h b^(2p)) E_b with nu_h = rate / (n/3)^(2p). The bins are round(|k|); over a
+    bin of width one, |k|^(2p) averages to b^(2p) (1 + O(p^2 / b^2)), and the hyperviscous loss
+    sits at b in the hundreds."""
+    b = np.arange(spectrum.size, dtype=np.float64)
+    p = params["hyper_p"]
+    nu = params["viscosity"] / (n / 3.0) ** (2 * p) if params["viscosity"] > 0 else 0.0
+    return float(np.sum(2.0 * (params["drag"] + nu * b ** (2 * p)) * spectrum))
+
+
+def spectral_slope(spectrum, kmin, kmax):
+    """Least-squares slope of log E(k) against log k over the bins kmin..kmax."""
+    b = np.arange(spectrum.size)
+    sel = (b >= kmin) & (b <= kmax) & (spectrum > 0)
+    return float(np.polyfit(np.log(b[sel]), np.log(spectrum[sel]), 1)[0])
+
+
+def anisotropy(s2t, w, n):
+    """<u^2> / <v^2> of the physical velocity of the spectrum w (FP64 on the host)."""
+    k = np.fft.fftfreq(n, 1.0 / n)
+    ky, kx = np.meshgrid(k, k, indexing="ij")
+    k2 = kx * kx + ky * ky
+    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    u = np.fft.ifft2(1j * ky * inv * w).real
+    v = np.fft.ifft2(-1j * kx * inv * w).real
+    return float(np.mean(u * u) / np.mean(v * v))
+
+
+def run_forced_horizon(s2t, n, steps, every=25):
+    """The forced beta-plane workload from rest for `steps` CFL steps, sampling the FP64
+    diagnostics every `every` steps. Returns the time series of E, Z and the dissipation rate,
+    the energy budget (E(T) against the injected eps T minus the integrated dissipation,
+    trapezoid in time over the samples), the final zonal-mean u(y), E(k), its slope over the
+    enstrophy-cascade range and the velocity anisotropy <u^2>/<v^2>."""
+    params = workload_params(n)
+    eps = 1e-3
+    ts, es, zs, ds = [0.0], [0.0], [0.0], [0.0]
+    t0 = time.perf_counter()
+    with s2t.Solver(n) as s:
+        s.configure(**params)
+        s.seed_spectrum(k0=1.0, amplitude=0.0, seed=1)  # from rest
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            s.step(k)
+            done += k
+            d = s.diagnostics()
+            if d["nonfinite"]:
+                raise RuntimeError("non-finite state at step %d" % done)
+            ts.append(d["time"])
+            es.append(d["energy"])
+            zs.append(d["enstrophy"])
+            ds.append(dissipation_rate(d["spectrum"], n, params))
+        wall = time.perf_counter() - t0
+        w = s.read_state()
+    t = np.asarray(ts)
+    lost = float(np.sum(0.5 * (np.asarray(ds[1:]) + np.asarray(ds[:-1])) * np.diff(t)))
+    injected = eps * t[-1]
+    return {"n": n, "steps": steps, "time": float(t[-1]), "wall_s": round(wall, 2),
+            "final_dt": d["dt"], "energy": d["energy"], "enstrophy": d["enstrophy"],
+            "budget": {"injected": injected, "dissipated": lost,
+                       "predicted_energy": injected - lost, "energy": d["energy"],
+                       "residual_over_injected": (d["energy"] - (injected - lost)) / injected},
+            "spectrum_slope_k64_k256": spectral_slope(d["spectrum"], 64, 256),
+            "anisotropy_u2_v2": anisotropy(s2t, w, n),
+            "series": {"time": ts, "energy": es, "enstrophy": zs, "dissipation": ds},
+            "zonal_mean_u": [float("%.6g" % x) for x in d["zonal"]],
+            "energy_spectrum": [float("%.6g" % x) for x in d["spectrum"]]}
+
+
+def run_decaying_horizon(s2t, steps, every=20):
+    """The 1024^2 decaying workload with dye on 2048^2 for `steps` CFL steps: the enstrophy and
+    energy series (every `every` steps), the E(k) slope over k in [32, 160] at the end and the
+    dye's range."""
+    n = 1024
+    params = dict(beta=0.0, drag=0.0, viscosity=6.0, hyper_p=4.0, force_amp=0.0, force_k=32.0,
+                  force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
+    ts, es, zs = [], [], []
+    t0 = time.perf_counter()
+    with s2t.Solver(n, dye_scale=2) as s:
+        m = s.dye_size()
+        s.configure(**params)
+        s.seed_bandlimited(8.0, 16.0, 6.0, seed=2)
+        s.write_dye(blob_dye(m, seed=2))
+        d = s.diagnostics()
+        ts.append(d["time"]), es.append(d["energy"]), zs.append(d["enstrophy"])
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            s.step(k, dye=True)
+            done += k
+            d = s.diagnostics()
+            if d["nonfinite"]:
+                raise RuntimeError("non-finite state at step %d" % done)
+            ts.append(d["time"]), es.append(d["energy"]), zs.append(d["enstrophy"])
+        wall = time.perf_counter() - t0
+        dye = s.read_dye().astype(np.float64)
+    return {"n": n, "dye": m, "steps": steps, "time": ts[-1], "wall_s": round(wall, 2),
+            "final_dt": d["dt"], "enstrophy_decay": 1.0 - zs[-1] / zs[0],
+            "energy_decay": 1.0 - es[-1] / es[0],
+            "spectrum_slope_k32_k160": spectral_slope(d["spectrum"], 32, 160),
+            "dye_min": float(dye[..., :3].min()), "dye_max": float(dye[..., :3].max()),
+            "dye_alpha_one": bool(np.all(dye[..., 3] == 1.0)),
+            "series": {"time": ts, "energy": es, "enstrophy": zs},
+            "energy_spectrum": [float("%.6g" % x) for x in d["spectrum"]]}
+
+
+def bench_horizons(s2t, out=None):
+    """C2 to 2000 steps and C3 to 5000 steps at 2048^2 and 4096^2: the workloads' own horizons
+    and outputs (zonal-mean u and E(k)), with the statistics their physics fixes."""
+    res = {"decaying_1024": run_decaying_horizon(s2t, 2000)}
+    print("decaying 1024^2: t %.3f, slope %.2f" % (res["decaying_1024"]["time"],
+          res["decaying_1024"]["spectrum_slope_k32_k160"]), file=sys.stderr, flush=True)
+    for n in (2048, 4096):
+        r = run_forced_horizon(s2t, n, 5000)
+        res["forced_%d" % n] = r
+        print("forced %d^2: t %.3f, budget residual %.4f, slope %.2f, anisotropy %.2f"
+              % (n, r["time"], r["budget"]["residual_over_injected"],
+                 r["spectrum_slope_k64_k256"], r["anisotropy_u2_v2"]), file=sys.stderr, flush=True)
+    if out:
+        with open(out, "w") as f:
+            json.dump(res, f, indent=1)
+    summary = {k: {kk: vv for kk, vv in v.items()
+                   if kk not in ("series", "zonal_mean_u", "energy_spectrum")}
+               for k, v in res.items()}
+    return {"metric": "long-horizon statistics of the decaying and forced workloads",
+            "config": {"workload": "decaying 1024^2 x 2000 steps with dye 2048^2; forced "
+                                   "beta-plane 2048^2 and 4096^2 x 5000 steps from rest"},
+            "results": summary}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,7 +436,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", choices=["step", "ensemble", "serve", "decaying"],
+    ap.add_argument("--workload", choices=["step", "ensemble", "serve", "decaying", "horizons"],
                     default="step",
                     help="step: one 4096^2 instance (default); ensemble: 64 mixed-size members; "
                          "serve: Poisson arrivals into the serving loop; decaying: 1024^2 "
@@ -314,6 +446,8 @@ def main(argv=None):
     ap.add_argument("--rates", type=str, default="",
                     help="serve: comma-separated arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=120, help="serve: requests per rate")
+    ap.add_argument("--out", type=str, default="",
+                    help="horizons: also write the full series and profiles to this JSON file")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args(argv)
@@ -330,6 +464,9 @@ def main(argv=None):
         return 0
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(s2t, args.steps, args.warmup, args.plain_steps)))
+        return 0
+    if args.workload == "horizons":
+        print(json.dumps(bench_horizons(s2t, args.out)))
         return 0
     if args.workload == "decaying":
         print(json.dumps(bench_decaying(s2t, args.steps, args.warmup)))

@@ -16,6 +16,7 @@
 #ifndef S2T_H
 #define S2T_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -162,6 +163,12 @@ int s2t_set_clock(s2t_solver* h, double time, uint32_t step_index);
 int s2t_step(s2t_solver* h, int nsteps, uint32_t flags);
 /* Wait for all queued work on the solver's stream. */
 int s2t_sync(s2t_solver* h);
+/* Solver.encodePrime (solver.js:475-486): the velocity planes of the current state (every step
+ * primes a replaced state by itself; this does it now). Enqueues only. */
+int s2t_prime(s2t_solver* h);
+/* Solver.primed: 1 once the current state's velocity planes exist (after a prime or a step),
+ * 0 after seeding or writing the state. */
+int s2t_primed(s2t_solver* h, int* primed);
 
 /* ------------------------------------------------------------------------------------------ */
 /* diagnostics and dye                                                                        */
@@ -171,6 +178,16 @@ int s2t_sync(s2t_solver* h);
  * spectrum[n/2+1] = E(k) in bins round(|k|) (reference/spectral2d.py:98-104), zonal[n] =
  * zonal-mean u per row (spectral2d.py:106-108). Either array may be NULL. Waits. */
 int s2t_diagnostics(s2t_solver* h, s2t_diag* out, double* spectrum, double* zonal);
+/* Solver.encodeDiagnostics (solver.js:547-558): the FP64 diagnostics of the current state and a
+ * snapshot of its clock into the solver's device record. Enqueues only. */
+int s2t_encode_diagnostics(s2t_solver* h);
+/* Solver.copyDiagnostics (solver.js:560-568): the last encoded record to pinned host memory.
+ * Enqueues only; S2T_ESTATE when nothing was encoded. */
+int s2t_copy_diagnostics(s2t_solver* h);
+/* readDiagnostics (solver.js:570-608) of the copied record: waits for that copy only (not for
+ * work queued after it). Fills the fields of s2t_diagnostics; S2T_ESTATE when nothing was copied
+ * since the last read. */
+int s2t_read_copied_diagnostics(s2t_solver* h, s2t_diag* out, double* spectrum, double* zonal);
 /* Solver.paintDye(pattern) (solver.js:446-473): 0 = 'none', 1 = 'bands'. */
 int s2t_paint_dye(s2t_solver* h, int pattern);
 /* Dye texture side M = n * dye_scale. */
@@ -183,8 +200,13 @@ int s2t_write_dye(s2t_solver* h, const uint16_t* rgba);
  * s2t_step with S2T_STEP_DYE writes it from each step's initial field. */
 int s2t_write_velocity(s2t_solver* h, const float* uv);
 /* One dye update (advect_fwd, advect_back, advect_corr; src/shaders/dye.wgsl:58-113) with the
- * given dt and the current velocity, splats and fade. */
+ * current velocity, splats and fade: Solver.encodeDye (solver.js:523-536) when dt <= 0, which
+ * takes the step of the last solver step from the device (dtbuf, dye.wgsl:63); a given dt > 0 is
+ * used as is. Enqueues only. */
 int s2t_dye_step(s2t_solver* h, double dt);
+/* Blank all three dye textures and reset their parity (part of Solver.reset, solver.js:420-436,
+ * before paintDye). Waits. */
+int s2t_reset_dye(s2t_solver* h);
 
 /* ------------------------------------------------------------------------------------------ */
 /* test hooks: the product kernels on host data (the reference harness reaches the same stages  */
@@ -194,6 +216,20 @@ int s2t_dye_step(s2t_solver* h, double dt);
 /* Fft2D.forward / inverse on a full complex n x n field (fft.js:81-100): row pass then column
  * pass, forward unscaled, inverse scaled by 1/n per pass. in and out may alias. */
 int s2t_fft2(s2t_solver* h, const float* in, float* out, int inverse);
+/* Fft2D.run (fft.js:81-92) on a device buffer: the n x n complex field at data_dev (2*n*n floats,
+ * fixture layout, device memory) transformed in place on the solver's stream, row pass then
+ * column pass, forward unscaled, inverse scaled by 1/n per pass. Enqueues only. */
+int s2t_fft2_device(s2t_solver* h, float* data_dev, int inverse);
+/* Device buffers for the Fft2D.register route (GPUDevice.createBuffer, queue.writeBuffer,
+ * copyBufferToBuffer and a mapped read, fft.js:77-79): zero-filled allocation, free, and
+ * host-blocking copies. They are not ordered against a solver's stream: call s2t_sync on a
+ * solver whose queued work touches the buffer first. */
+int s2t_buffer_create(size_t bytes, void** dev);
+int s2t_buffer_destroy(void* dev);
+int s2t_buffer_write(void* dev, size_t offset, const void* src, size_t bytes);
+int s2t_buffer_read(const void* dev, size_t offset, void* dst, size_t bytes);
+int s2t_buffer_copy(void* dst, size_t dst_offset, const void* src, size_t src_offset,
+                    size_t bytes);
 /* velocity_pack (solver.wgsl:21-40) on a full spectrum: packA = w + i u_hat, packB = v_hat. */
 int s2t_velocity_pack(s2t_solver* h, const float* w, float* pack_a, float* pack_b);
 /* assemble (solver.wgsl:56-95) on a full product spectrum z and pointer-forcing spectrum fhat
@@ -299,6 +335,10 @@ int s2t_server_cancel(s2t_server* s, uint64_t id);
  * s2t_server_read_state; it must not call s2t_server_destroy. NULL removes it. */
 typedef void (*s2t_server_callback)(uint64_t id, int status, void* user);
 int s2t_server_set_callback(s2t_server* s, s2t_server_callback cb, void* user);
+/* Failure injection, with no counterpart in the reference, for callers that test their handling
+ * of a failed server: the worker fails (as on a device error) at the admission after
+ * `after_admissions` more admissions counted from the server's start; < 0 turns it off. */
+int s2t_server_inject_failure(s2t_server* s, int after_admissions);
 /* Wait until every submitted request is done, rejected or cancelled (timeout_s <= 0: no limit). */
 int s2t_server_drain(s2t_server* s, double timeout_s);
 int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);

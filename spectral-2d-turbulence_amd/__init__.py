@@ -45,7 +45,10 @@ EXPORTS = [
     "s2t_batch_sync", "s2t_batch_autotune", "s2t_batch_column_width",
     "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
     "s2t_server_read_state", "s2t_server_cancel", "s2t_server_set_callback", "s2t_server_drain",
-    "s2t_server_get_stats", "s2t_fill_spectrum_noalloc",
+    "s2t_server_get_stats", "s2t_fill_spectrum_noalloc", "s2t_reset_dye", "s2t_prime",
+    "s2t_primed", "s2t_encode_diagnostics", "s2t_copy_diagnostics",
+    "s2t_read_copied_diagnostics", "s2t_buffer_create", "s2t_buffer_destroy", "s2t_buffer_write",
+    "s2t_buffer_read", "s2t_buffer_copy", "s2t_fft2_device", "s2t_server_inject_failure",
 ]
 SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4, 5
@@ -174,6 +177,19 @@ def load(path=None):
         "s2t_server_drain": (ip, [vp, dp]),
         "s2t_server_get_stats": (ip, [vp, ctypes.POINTER(ServerStats)]),
         "s2t_fill_spectrum_noalloc": (ip, [ip, dp, dp, ctypes.c_uint32, vp, vp]),
+        "s2t_reset_dye": (ip, [vp]),
+        "s2t_prime": (ip, [vp]),
+        "s2t_primed": (ip, [vp, ctypes.POINTER(ip)]),
+        "s2t_encode_diagnostics": (ip, [vp]),
+        "s2t_copy_diagnostics": (ip, [vp]),
+        "s2t_read_copied_diagnostics": (ip, [vp, ctypes.POINTER(Diag), vp, vp]),
+        "s2t_buffer_create": (ip, [ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "s2t_buffer_destroy": (ip, [vp]),
+        "s2t_buffer_write": (ip, [vp, ctypes.c_size_t, vp, ctypes.c_size_t]),
+        "s2t_buffer_read": (ip, [vp, ctypes.c_size_t, vp, ctypes.c_size_t]),
+        "s2t_buffer_copy": (ip, [vp, ctypes.c_size_t, vp, ctypes.c_size_t, ctypes.c_size_t]),
+        "s2t_fft2_device": (ip, [vp, vp, ip]),
+        "s2t_server_inject_failure": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -332,8 +348,41 @@ class Solver:
         a = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1))
         _check(self.lib, self.lib.s2t_write_velocity(self.h, _ptr(a)))
 
-    def dye_step(self, dt):
+    def dye_step(self, dt=0.0):
+        """One dye update (dt <= 0: the last step's dt on the device); enqueues only."""
         _check(self.lib, self.lib.s2t_dye_step(self.h, float(dt)))
+
+    def reset_dye(self):
+        _check(self.lib, self.lib.s2t_reset_dye(self.h))
+
+    def prime(self):
+        _check(self.lib, self.lib.s2t_prime(self.h))
+
+    @property
+    def primed(self):
+        v = ctypes.c_int()
+        _check(self.lib, self.lib.s2t_primed(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def encode_diagnostics(self):
+        _check(self.lib, self.lib.s2t_encode_diagnostics(self.h))
+
+    def copy_diagnostics(self):
+        _check(self.lib, self.lib.s2t_copy_diagnostics(self.h))
+
+    def read_copied_diagnostics(self):
+        d = Diag()
+        spec = np.empty(self.n // 2 + 1, np.float64)
+        zonal = np.empty(self.n, np.float64)
+        _check(self.lib, self.lib.s2t_read_copied_diagnostics(self.h, ctypes.byref(d),
+                                                              _ptr(spec), _ptr(zonal)))
+        return dict(energy=d.energy, enstrophy=d.enstrophy, dt=d.dt, maxSpeed=d.max_speed,
+                    time=d.time, stepIndex=d.step_index, nonfinite=d.nonfinite,
+                    spectrum=spec, zonal=zonal)
+
+    def fft2_device(self, dev_ptr, inverse=False):
+        """In-place 2-D FFT of an n x n complex device buffer (DeviceBuffer.ptr); enqueues."""
+        _check(self.lib, self.lib.s2t_fft2_device(self.h, dev_ptr, 1 if inverse else 0))
 
     # test hooks
     def fft2(self, z, inverse=False):
@@ -441,6 +490,44 @@ def bandlimited_spectrum(n, kmin=8.0, kmax=16.0, amplitude=6.0, seed=2):
     _check(lib, lib.s2t_bandlimited_spectrum(int(n), float(kmin), float(kmax), float(amplitude),
                                              int(seed) & 0xFFFFFFFF, _ptr(a)))
     return from_interleaved(a, n)
+
+
+class DeviceBuffer:
+    """Device memory for the Fft2D.register route (s2t_buffer_*): zero-filled at creation;
+    write / read / copy_from are host-blocking and not ordered against a solver's stream."""
+
+    def __init__(self, nbytes):
+        self.lib = load()
+        p = ctypes.c_void_p()
+        _check(self.lib, self.lib.s2t_buffer_create(int(nbytes), ctypes.byref(p)))
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    def write(self, data, offset=0):
+        a = np.ascontiguousarray(data)
+        _check(self.lib, self.lib.s2t_buffer_write(self.ptr, int(offset), _ptr(a), a.nbytes))
+
+    def read(self, dtype=np.float32, offset=0, nbytes=None):
+        nbytes = self.nbytes - offset if nbytes is None else int(nbytes)
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        _check(self.lib, self.lib.s2t_buffer_read(self.ptr, int(offset), _ptr(out), out.nbytes))
+        return out
+
+    def copy_from(self, src, src_offset=0, dst_offset=0, nbytes=None):
+        nbytes = src.nbytes - src_offset if nbytes is None else int(nbytes)
+        _check(self.lib, self.lib.s2t_buffer_copy(self.ptr, int(dst_offset), src.ptr,
+                                                  int(src_offset), nbytes))
+
+    def close(self):
+        if self.ptr:
+            self.lib.s2t_buffer_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
@@ -597,6 +684,10 @@ class Server:
 
     def drain(self, timeout_s=0.0):
         _check(self.lib, self.lib.s2t_server_drain(self.h, float(timeout_s)))
+
+    def inject_failure(self, after_admissions):
+        """Make the worker fail at the admission after `after_admissions` admissions."""
+        _check(self.lib, self.lib.s2t_server_inject_failure(self.h, int(after_admissions)))
 
     def stats(self):
         st = ServerStats()

@@ -281,11 +281,106 @@ int s2t_write_velocity(s2t_solver* h, const float* uv) {
 int s2t_dye_step(s2t_solver* h, double dt) {
   SOLVER(h);
   if (!s->vel) return api_bad("s2t_dye_step: no velocity (s2t_write_velocity or a dye step)");
+  if (!std::isfinite(dt)) return api_bad("s2t_dye_step: dt must be finite");
   S2T_API_BEGIN
-  dye_step(s->stream, s->dye, s->dye_parity, s->vel, nullptr, dt, s->phys_slot[0].dye_fade,
-           s->splats, s->n, s->m);
+  // dt <= 0: the step of the last solver step, read by the kernels on the device
+  dye_step(s->stream, s->dye, s->dye_parity, s->vel, dt > 0 ? nullptr : &s->inst[0].st, dt,
+           s->phys_slot[0].dye_fade, s->splats, s->n, s->m);
   s->dye_parity = 1 - s->dye_parity;
+  S2T_API_END
+}
+
+int s2t_reset_dye(s2t_solver* h) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  for (int i = 0; i < 3; ++i)
+    if (s->dye[i]) fill_u32(s->stream, s->dye[i], (size_t)s->m * s->m * 2, 0u);
+  s->dye_parity = 0;
   sync(s);
+  S2T_API_END
+}
+
+int s2t_prime(s2t_solver* h) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  ensure_primed(s);
+  S2T_API_END
+}
+
+int s2t_primed(s2t_solver* h, int* primed) {
+  SOLVER(h);
+  if (!primed) return api_bad("s2t_primed: out is NULL");
+  *primed = s->primed_slot[0] ? 1 : 0;
+  return S2T_OK;
+}
+
+int s2t_encode_diagnostics(s2t_solver* h) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  encode_diag(s);
+  S2T_API_END
+}
+
+int s2t_copy_diagnostics(s2t_solver* h) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  copy_diag(s);
+  S2T_API_END
+}
+
+int s2t_read_copied_diagnostics(s2t_solver* h, s2t_diag* out, double* spectrum, double* zonal) {
+  SOLVER(h);
+  S2T_API_BEGIN
+  read_copied_diag(s, out, spectrum, zonal);
+  S2T_API_END
+}
+
+int s2t_buffer_create(size_t bytes, void** dev) {
+  if (!dev) return api_bad("s2t_buffer_create: out is NULL");
+  *dev = nullptr;
+  if (bytes == 0) return api_bad("s2t_buffer_create: size must be positive");
+  S2T_API_BEGIN
+  hipchk(hipMalloc(dev, bytes), "hipMalloc buffer");
+  hipchk(hipMemset(*dev, 0, bytes), "hipMemset buffer");
+  S2T_API_END
+}
+
+int s2t_buffer_destroy(void* dev) {
+  if (dev) (void)hipFree(dev);
+  return S2T_OK;
+}
+
+int s2t_buffer_write(void* dev, size_t offset, const void* src, size_t bytes) {
+  if (!dev || (!src && bytes)) return api_bad("s2t_buffer_write: NULL pointer");
+  S2T_API_BEGIN
+  hipchk(hipMemcpy(static_cast<char*>(dev) + offset, src, bytes, hipMemcpyHostToDevice),
+         "hipMemcpy H2D");
+  S2T_API_END
+}
+
+int s2t_buffer_read(const void* dev, size_t offset, void* dst, size_t bytes) {
+  if (!dev || (!dst && bytes)) return api_bad("s2t_buffer_read: NULL pointer");
+  S2T_API_BEGIN
+  hipchk(hipMemcpy(dst, static_cast<const char*>(dev) + offset, bytes, hipMemcpyDeviceToHost),
+         "hipMemcpy D2H");
+  S2T_API_END
+}
+
+int s2t_buffer_copy(void* dst, size_t dst_offset, const void* src, size_t src_offset,
+                    size_t bytes) {
+  if (!dst || !src) return api_bad("s2t_buffer_copy: NULL pointer");
+  S2T_API_BEGIN
+  hipchk(hipMemcpy(static_cast<char*>(dst) + dst_offset,
+                   static_cast<const char*>(src) + src_offset, bytes, hipMemcpyDeviceToDevice),
+         "hipMemcpy D2D");
+  S2T_API_END
+}
+
+int s2t_fft2_device(s2t_solver* h, float* data_dev, int inverse) {
+  SOLVER(h);
+  if (!data_dev) return api_bad("s2t_fft2_device: NULL buffer");
+  S2T_API_BEGIN
+  s->ops->fft2(s, reinterpret_cast<float2*>(data_dev), inverse ? 1 : 0);
   S2T_API_END
 }
 

@@ -242,6 +242,9 @@ void solver_free(Solver* s) {
                   s->diag_bins, s->diag_zonal, s->seed_sum, s->dye[0], s->dye[1], s->dye[2]};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (s->enc_diag) (void)hipFree(s->enc_diag);
+  if (s->enc_host) (void)hipHostFree(s->enc_host);
+  if (s->enc_event) (void)hipEventDestroy(s->enc_event);
   for (int i = 0; i < s->cap_ev; ++i) {
     (void)hipEventDestroy(s->evs[i].a);
     (void)hipEventDestroy(s->evs[i].b);
@@ -334,6 +337,65 @@ void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zon
   }
   if (spectrum) memcpy(spectrum, bins.data(), nb * sizeof(double));
   if (zonal) d2h(s, zonal, s->diag_zonal, s->n * sizeof(double));
+}
+
+// Solver.encodeDiagnostics (solver.js:547-558): the FP64 diagnostics of slot 0's current state
+// and a snapshot of its clock, into the solver's device record, enqueued only. Layout of the
+// record: bins (2 nb + 2 doubles: E(k), Z(k), E, Z), the zonal mean (n doubles), the DevState.
+static size_t enc_bytes(const Solver* s) {
+  const int nb = s->n / 2 + 1;
+  return (size_t)(2 * nb + 2 + s->n) * sizeof(double) + sizeof(DevState);
+}
+
+void encode_diag(Solver* s) {
+  const int nb = s->n / 2 + 1;
+  if (!s->enc_diag) s->enc_diag = dalloc<char>(enc_bytes(s), "diagnostics record");
+  double* bins = reinterpret_cast<double*>(s->enc_diag);
+  double* zonal = bins + 2 * nb + 2;
+  diagnostics(s->stream, slot_state(s, 0), s->wtab, bins, zonal, s->n);
+  hipchk(hipMemcpyAsync(zonal + s->n, &s->inst[0].st, sizeof(DevState), hipMemcpyDeviceToDevice,
+                        s->stream),
+         "hipMemcpyAsync clock snapshot");
+}
+
+// Solver.copyDiagnostics (solver.js:560-568): the last encoded record to pinned host memory,
+// enqueued; an event marks its arrival
+void copy_diag(Solver* s) {
+  if (!s->enc_diag) throw Fail(S2T_ESTATE, "copy_diagnostics: no diagnostics were encoded");
+  if (!s->enc_host) {
+    void* p = nullptr;
+    hipchk(hipHostMalloc(&p, enc_bytes(s), 0), "hipHostMalloc diagnostics staging");
+    s->enc_host = static_cast<char*>(p);
+  }
+  if (!s->enc_event)
+    hipchk(hipEventCreateWithFlags(&s->enc_event, hipEventDisableTiming), "hipEventCreate");
+  hipchk(hipMemcpyAsync(s->enc_host, s->enc_diag, enc_bytes(s), hipMemcpyDeviceToHost, s->stream),
+         "hipMemcpyAsync diagnostics record");
+  hipchk(hipEventRecord(s->enc_event, s->stream), "hipEventRecord");
+  s->enc_copied = true;
+}
+
+// readDiagnostics (solver.js:570-608) of the copied record: waits for the copy only
+void read_copied_diag(Solver* s, s2t_diag* out, double* spectrum, double* zonal) {
+  if (!s->enc_copied) throw Fail(S2T_ESTATE, "read_copied_diagnostics: nothing was copied");
+  hipchk(hipEventSynchronize(s->enc_event), "hipEventSynchronize");
+  s->enc_copied = false;
+  const int nb = s->n / 2 + 1;
+  const double* bins = reinterpret_cast<const double*>(s->enc_host);
+  const double* zon = bins + 2 * nb + 2;
+  DevState d;
+  memcpy(&d, zon + s->n, sizeof(DevState));
+  if (out) {
+    out->energy = bins[2 * nb];
+    out->enstrophy = bins[2 * nb + 1];
+    out->dt = d.dt;
+    out->max_speed = d.speed;
+    out->time = d.t;
+    out->step_index = d.step;
+    out->nonfinite = d.nonfinite;
+  }
+  if (spectrum) memcpy(spectrum, bins, nb * sizeof(double));
+  if (zonal) memcpy(zonal, zon, s->n * sizeof(double));
 }
 
 // ---------------------------------------------------------------------------------------------

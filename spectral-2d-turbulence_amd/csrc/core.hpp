@@ -45,6 +45,11 @@ void set_budget_slot(Solver* s, int slot, int steps);
 void set_clock_slot(Solver* s, int slot, double t, unsigned step);
 DevState read_clock_slot(Solver* s, int slot);
 
+// encodeDiagnostics / copyDiagnostics / readDiagnostics of slot 0 (enqueue, enqueue, wait)
+void encode_diag(Solver* s);
+void copy_diag(Solver* s);
+void read_copied_diag(Solver* s, s2t_diag* out, double* spectrum, double* zonal);
+
 void ensure_primed(Solver* s);
 void step_once(Solver* s, bool dye);
 // column-group width tuning (s2t_autotune): returns the chosen width, also kept in s->cols_c

@@ -129,9 +129,10 @@ struct Server {
   double t_first = 0, t_last = 0;
   uint64_t submitted = 0, completed = 0, rejected = 0, cancelled = 0;
   std::string fatal;
-  // failure injection for tests of the failure path, from the environment at creation
-  // (S2T_SERVER_FAIL_AFTER = k: the worker fails at the admission after k admissions)
-  int fail_after = -1, admitted = 0;
+  // failure injection (s2t_server_inject_failure = k: the worker fails at the admission after k
+  // admissions), so that callers can test their handling of a failed server
+  std::atomic<int> fail_after{-1};
+  int admitted = 0;
   s2t_server_callback callback = nullptr;
   void* callback_user = nullptr;
 };
@@ -283,7 +284,7 @@ static void worker_loop(Server* sv) {
       for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
       for (auto& tr : take) {
         if (sv->fail_after >= 0 && sv->admitted >= sv->fail_after)
-          throw Fail(S2T_EHIP, "worker failure injected by S2T_SERVER_FAIL_AFTER");
+          throw Fail(S2T_EHIP, "worker failure injected by s2t_server_inject_failure");
         Class& c = sv->cls[tr.first];
         int slot = 0;
         while (c.slot_req[slot] >= 0) ++slot;
@@ -408,7 +409,6 @@ int s2t_server_create(const int* sizes, const int* slots, int nclasses, double b
   Server* sv = new Server();
   S2T_API_BEGIN
   sv->budget = (size_t)budget_bytes;
-  if (const char* f = std::getenv("S2T_SERVER_FAIL_AFTER")) sv->fail_after = std::atoi(f);
   for (int i = 0; i < nclasses; ++i) {
     if (sv->cls.count(sizes[i])) throw Fail(S2T_EINVAL, "duplicate class size");
     Class c;
@@ -507,6 +507,13 @@ int s2t_server_cancel(s2t_server* h, uint64_t id) {
   }
   g_err = "request already finished";
   return S2T_ESTATE;
+}
+
+int s2t_server_inject_failure(s2t_server* h, int after_admissions) {
+  Server* sv = reinterpret_cast<Server*>(h);
+  if (!sv) return api_bad("s2t_server_inject_failure: null server");
+  sv->fail_after = after_admissions < 0 ? -1 : after_admissions;
+  return S2T_OK;
 }
 
 int s2t_server_set_callback(s2t_server* h, s2t_server_callback cb, void* user) {

@@ -62,6 +62,12 @@ struct Solver {
   double* diag_zonal = nullptr;
   double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation
   ushort4* dye[3] = {nullptr, nullptr, nullptr};
+  // encodeDiagnostics / copyDiagnostics: the record of the last encode on the device (bins,
+  // zonal mean, clock), its pinned host copy and the event that marks the copy, lazily
+  char* enc_diag = nullptr;
+  char* enc_host = nullptr;
+  hipEvent_t enc_event = nullptr;
+  bool enc_copied = false;
 
   // host mirror of the configuration of each slot
   std::vector<Phys> phys_slot;

@@ -3,14 +3,21 @@
 // (src/gpu/solver.js, src/gpu/fft.js, src/scenarios.js) over the N-API addon s2t_napi.node,
 // which wraps the C ABI of include/s2t.h.
 //
-// Differences from the WebGPU original, by design:
-//  * there is no GPUDevice or command encoder: `device` arguments are accepted and ignored, and the
-//    encode* methods enqueue directly on the solver's HIP stream (Solver.step(n) runs n steps);
-//  * the velocity prime (encodePrime) and the CFL reduction (encodeReduce) are part of each step;
-//  * buffers live on the device; the test-visible internals of the browser harness are offered as
-//    explicit methods (readState / writeState, velocityPack, assemble, rhs, Fft2D on host arrays).
+// The frame-loop surface is the reference's: encodePrime / encodeStep / encodeDye / encodeReduce /
+// encodeDiagnostics record work into a compute pass of the device in device.js, copyDiagnostics
+// into its command encoder, queue.submit runs it and readDiagnostics resolves with the copied
+// record (null when nothing was copied). Differences, by design:
+//  * the CFL reduction runs inside every step, from the step's initial field
+//    (reference/spectral2d.py:66-71), so encodeReduce records nothing;
+//  * encodeDye straight after encodeStep of the same solver in one pass fuses into that step (its
+//    row kernel writes the dye velocity); on its own it advects with the last dye velocity;
+//  * the solver's full-size buffers (buf.packA, packB, physZ, force) are device buffers for the
+//    Fft2D.register route; buf.state[k] reads and writes the current state in the fixture layout;
+//  * solver.step(n, {dye}) and the host-array hooks (readState / writeState, velocityPack,
+//    assemble, rhs, Fft2D on host arrays) remain as direct calls.
 
 const path = require('path');
+const { Device, DeviceBuffer, GPUBufferUsage, GPUMapMode, requestDevice } = require('./device.js');
 
 let native = null;
 function addon() {
@@ -43,22 +50,71 @@ const DEFAULT_CONFIG = Object.freeze({
   dyeFade: 0.999,
 });
 
+// buf.state[k] of the reference (the current spectrum, fixture layout) and buf.dt (a dt written
+// there becomes the solver's fixed step, as the reference harness writes dtbuf directly,
+// tests/index.html:188)
+class StateView {
+  constructor(solver) {
+    this.solver = solver;
+    this.size = solver.ncells * 8;
+    this.label = 'state';
+  }
+
+  _write(offset, bytes) {
+    if (offset !== 0 || bytes.byteLength !== this.size) throw new Error('write the whole state');
+    const f = new Float32Array(bytes.buffer.slice(bytes.byteOffset, bytes.byteOffset + this.size));
+    this.solver.writeState(f);
+  }
+
+  _copyTo(offset, dst, dstOffset, size) {
+    const st = this.solver.readState();
+    dst._write(dstOffset, new Uint8Array(st.buffer, offset, size));
+  }
+}
+
+class DtView {
+  constructor(solver) {
+    this.solver = solver;
+    this.size = 16;
+    this.label = 'dt';
+  }
+
+  _write(offset, bytes) {
+    const f = new Float32Array(bytes.buffer.slice(bytes.byteOffset, bytes.byteOffset + 4));
+    this.solver.configure({ fixedDt: f[0] });
+  }
+
+  _copyTo(offset, dst, dstOffset, size) {
+    const d = this.solver.diagnostics();
+    const f = new Float32Array([d.dt, d.maxSpeed, 0, 0]);
+    dst._write(dstOffset, new Uint8Array(f.buffer, offset, size));
+  }
+}
+
 class Solver {
-  constructor(handle, n, dyeScale) {
+  constructor(device, handle, n, dyeScale) {
+    this.device = device || new Device();
     this.handle = handle;
     this.n = n;
     this.ncells = n * n;
     this.m = n * dyeScale;
     this.nbins = n / 2 + 1;
     this.dyeScale = dyeScale;
+    this.phase = 0;
+    this.dyeParity = 0;
+    this.stepIndex = 0;
     this.simTime = 0;
+    this.readState_ = 'idle';
+    this.primed = false;
     this.disposed = false;
     this.cfg = { ...DEFAULT_CONFIG };
+    this.fft = new Fft2D(this.device, n, this);
+    this._buf = null;
   }
 
   // Solver.create(device, {n, dyeScale}) (solver.js:71-75)
   static async create(device, { n = 256, dyeScale = 2 } = {}) {
-    const s = new Solver(addon().create(n, dyeScale), n, dyeScale);
+    const s = new Solver(device, addon().create(n, dyeScale), n, dyeScale);
     s.configure(s.cfg);
     return s;
   }
@@ -66,6 +122,38 @@ class Solver {
   _h() {
     if (this.disposed) throw new Error('solver was destroyed');
     return this.handle;
+  }
+
+  _sync() {
+    if (!this.disposed) addon().sync(this.handle);
+  }
+
+  // the reference's named buffers (solver.js:88-104): full-size fields registered with this
+  // solver's Fft2D, created on first use
+  get buf() {
+    if (!this._buf) {
+      const bytes = this.ncells * 8;
+      const mk = (label) => this.device.createBuffer({ size: bytes, usage: GPUBufferUsage.STORAGE, label });
+      const b = {};
+      for (const name of ['packA', 'packB', 'physZ', 'force']) {
+        let made = null;
+        Object.defineProperty(b, name, {
+          enumerable: true,
+          get: () => {
+            if (!made) {
+              made = mk(name);
+              this.fft.register(name, made);
+            }
+            return made;
+          },
+        });
+      }
+      const state = new StateView(this);
+      b.state = [state, state, state];
+      b.dt = new DtView(this);
+      this._buf = b;
+    }
+    return this._buf;
   }
 
   // configure(cfg) (solver.js:306-324); cfg.fixedDt > 0 replaces the CFL step
@@ -79,6 +167,12 @@ class Solver {
     return addon().nuH(this._h());
   }
 
+  get paramsF() {
+    return Float32Array.from([this.cfg.beta, this.cfg.drag, this.nuH, this.cfg.hyperP,
+      this.cfg.forceAmp, this.cfg.forceK, this.cfg.forceWidth, 0, (2 * Math.PI) / this.n,
+      this.cfg.cfl, this.cfg.dtMin, this.cfg.dtMax]);
+  }
+
   // setSplats(list) (solver.js:326-345), clamped to MAX_SPLATS
   setSplats(list) {
     addon().setSplats(this._h(), list.slice(0, MAX_SPLATS));
@@ -87,60 +181,96 @@ class Solver {
   // seed({k0, amplitude, seed}) (solver.js:347-359)
   seed({ k0 = 14, amplitude = 6, seed = 1 } = {}) {
     addon().seedSpectrum(this._h(), k0, amplitude, seed);
+    this.phase = 0;
+    this.primed = false;
   }
 
   // seedPhysical(values) (solver.js:361-382)
   seedPhysical(values) {
     const f = values instanceof Float32Array ? values : Float32Array.from(values);
     addon().seedPhysical(this._h(), f, this.n);
+    this.phase = 0;
+    this.primed = false;
   }
 
-  // reset(opts) (solver.js:420-444)
+  // reset(opts) (solver.js:420-444): blank dye, then the pattern; clock at zero, dt = dtMax
   reset(opts = {}) {
+    this.stepIndex = 0;
     this.simTime = 0;
+    this.dyeParity = 0;
     if (opts.field) {
       this.seedPhysical(opts.field);
     } else {
       this.seed(opts);
     }
+    addon().resetDye(this._h());
     this.paintDye(opts.dyePattern !== undefined ? opts.dyePattern : 'bands');
+    this.primed = false;
   }
 
-  // paintDye(pattern) (solver.js:446-473)
+  // paintDye(pattern) (solver.js:446-473); 'none' leaves the dye as it is
   paintDye(pattern) {
-    addon().paintDye(this._h(), pattern === 'none' ? 0 : 1);
+    if (pattern === 'none') return;
+    addon().paintDye(this._h(), 1);
   }
 
-  // encodePrime: the prime is implicit in every step (kept for API parity)
-  encodePrime() {}
-
-  // encodeStep + encodeReduce (+ encodeDye when dye is true) for `count` steps
-  step(count = 1, { dye = false } = {}) {
-    addon().step(this._h(), count, !!dye);
+  // encodePrime(pass) (solver.js:475-486): the velocity of the current state
+  encodePrime(pass) {
+    pass._record(this, () => addon().prime(this._h()));
+    this.primed = true;
   }
 
-  // the same steps on a worker thread: the Promise resolves once the device has finished them
-  // (calls on the solver meanwhile wait for them)
-  stepAsync(count = 1, { dye = false } = {}) {
-    return addon().stepAsync(this._h(), count, !!dye);
+  // encodeStep(pass) (solver.js:488-521): one IF-RK3 step with its CFL reduction
+  encodeStep(pass) {
+    const op = pass._record(this, () => addon().step(this._h(), 1, op.tag === 'step+dye'), 'step');
+    this.phase = (this.phase + 1) % 3;
+    this.stepIndex += 1;
+    this.primed = true;
   }
 
-  encodeStep() {
-    this.step(1);
+  // encodeDye(pass) (solver.js:523-536): straight after this solver's step it fuses into it
+  encodeDye(pass) {
+    const last = pass._last();
+    if (last && last.owner === this && last.tag === 'step') {
+      last.tag = 'step+dye';
+    } else {
+      pass._record(this, () => addon().dyeStep(this._h(), 0), 'dye');
+    }
+    this.dyeParity = 1 - this.dyeParity;
   }
 
-  // column-group width of the column kernel, tuned on a scratch solver (s2t_autotune):
-  // {width, ms: {width: ms per step}}; results are the same at every width
-  autotune(steps = 5) {
-    return addon().autotune(this._h(), this.n, steps);
+  // encodeReduce(pass) (solver.js:538-545): the reduction is part of every step
+  encodeReduce() {}
+
+  // encodeDiagnostics(pass) (solver.js:547-558)
+  encodeDiagnostics(pass) {
+    pass._record(this, () => addon().encodeDiagnostics(this._h()), 'diagnostics');
   }
 
-  sync() {
-    addon().sync(this._h());
+  // copyDiagnostics(encoder) (solver.js:560-568): skipped while a read is outstanding
+  copyDiagnostics(encoder) {
+    if (this.readState_ !== 'idle') return;
+    encoder._record(this, () => addon().copyDiagnostics(this._h()), 'copy');
+    this.readState_ = 'copied';
   }
 
-  // readDiagnostics() (solver.js:570-608): FP64 record, or null after destroy()
+  // readDiagnostics() (solver.js:570-608): the copied record in FP64, or null when nothing was
+  // copied, the copy failed or the solver was destroyed
   async readDiagnostics() {
+    if (this.readState_ !== 'copied' || this.disposed) return null;
+    this.readState_ = 'mapping';
+    try {
+      const d = await addon().readCopiedDiagnosticsAsync(this.handle, this.n);
+      return this.disposed ? null : d;
+    } catch (e) {
+      return null;
+    } finally {
+      this.readState_ = 'idle';
+    }
+  }
+
+  // the diagnostics of the state after all calls made so far, on a worker thread
+  async diagnosticsAsync() {
     if (this.disposed) return null;
     const d = await addon().diagnosticsAsync(this._h(), this.n);
     this.simTime = d.time;
@@ -153,8 +283,33 @@ class Solver {
     return d;
   }
 
-  get stepIndex() {
-    return addon().diagnostics(this._h(), this.n).stepIndex;
+  // `count` steps (+ dye): encodeStep (+ encodeDye) + encodeReduce, enqueued at once
+  step(count = 1, { dye = false } = {}) {
+    addon().step(this._h(), count, !!dye);
+    this.stepIndex += count;
+    this.phase = (this.phase + count) % 3;
+    if (dye && count % 2) this.dyeParity = 1 - this.dyeParity;
+    if (count > 0) this.primed = true;
+  }
+
+  // the same steps on a worker thread: the Promise resolves once the device has finished them
+  // (calls on the solver meanwhile wait for them)
+  stepAsync(count = 1, { dye = false } = {}) {
+    this.stepIndex += count;
+    this.phase = (this.phase + count) % 3;
+    if (dye && count % 2) this.dyeParity = 1 - this.dyeParity;
+    if (count > 0) this.primed = true;
+    return addon().stepAsync(this._h(), count, !!dye);
+  }
+
+  // column-group width of the column kernel, tuned on a scratch solver (s2t_autotune):
+  // {width, ms: {width: ms per step}}; results are the same at every width
+  autotune(steps = 5) {
+    return addon().autotune(this._h(), this.n, steps);
+  }
+
+  sync() {
+    addon().sync(this._h());
   }
 
   // current dye texture as binary16 rgba (Solver.currentDye)
@@ -169,12 +324,14 @@ class Solver {
   writeState(spectrum) {
     const f = spectrum instanceof Float32Array ? spectrum : Float32Array.from(spectrum);
     addon().writeState(this._h(), f, this.n);
+    this.primed = false;
   }
 
   // simulated time and step index (resuming a checkpoint of readState() and the diagnostics'
   // time and stepIndex: the forcing phases follow the step index)
   setClock({ time = 0, stepIndex = 0 } = {}) {
     this.simTime = time;
+    this.stepIndex = stepIndex;
     addon().setClock(this._h(), time, stepIndex);
   }
 
@@ -196,34 +353,66 @@ class Solver {
 
   destroy() {
     this.disposed = true;
+    if (this._buf) {
+      for (const name of ['packA', 'packB', 'physZ', 'force']) {
+        const d = Object.getOwnPropertyDescriptor(this._buf, name);
+        if (d && d.get && this.fft.groups.has(name)) this.fft.groups.get(name).destroy();
+      }
+    }
     // the native handle is released by its finalizer when the last reference goes
     this.handle = null;
   }
 }
 
-// Fft2D (fft.js:33-101) on host arrays: forward unscaled, inverse scaled by 1/N per pass
+// Fft2D (fft.js:33-101): register(name, buffer) binds a device buffer; forward(pass, name) /
+// inverse(pass, name) record an in-place 2-D transform of it (row pass, then column pass; forward
+// unscaled, inverse 1/N per pass). forward(data) / inverse(data) on a host array (Float32Array in
+// the fixture layout) transform it at once and return the result.
 class Fft2D {
-  constructor(solver, n) {
-    this.solver = solver;
+  constructor(device, n, solver = null) {
+    this.device = device;
     this.n = n;
+    this.solver = solver;
+    this.groups = new Map();
+    this.handle = solver ? null : addon().create(n, 1);
   }
 
   static async create(device, n) {
     radixSchedule(n);
-    return new Fft2D(await Solver.create(device, { n, dyeScale: 1 }), n);
+    return new Fft2D(device || new Device(), n);
   }
 
-  run(data, inverse) {
+  _h() {
+    return this.solver ? this.solver._h() : this.handle;
+  }
+
+  _sync() {
+    if (this.solver) this.solver._sync();
+    else addon().sync(this.handle);
+  }
+
+  register(name, buffer) {
+    this.groups.set(name, buffer);
+  }
+
+  run(pass, name, inverse) {
+    if (!(pass && typeof pass._record === 'function')) return this._host(pass, inverse);
+    const buffer = this.groups.get(name);
+    if (!buffer) throw new Error(`buffer ${name} was never registered with the FFT`);
+    pass._record(this.solver || this, () => addon().fft2Device(this._h(), buffer._ext, this.n, !!inverse));
+  }
+
+  _host(data, inverse) {
     const f = data instanceof Float32Array ? data : Float32Array.from(data);
-    return addon().fft2(this.solver._h(), f, this.n, !!inverse);
+    return addon().fft2(this._h(), f, this.n, !!inverse);
   }
 
-  forward(data) {
-    return this.run(data, false);
+  forward(pass, name) {
+    return this.run(pass, name, false);
   }
 
-  inverse(data) {
-    return this.run(data, true);
+  inverse(pass, name) {
+    return this.run(pass, name, true);
   }
 }
 
@@ -445,6 +634,12 @@ class Server {
     return addon().serverStats(this.handle);
   }
 
+  // make the worker fail at the admission after `afterAdmissions` admissions (tests of a caller's
+  // handling of a failed server)
+  injectFailure(afterAdmissions) {
+    addon().serverInjectFailure(this.handle, afterAdmissions);
+  }
+
   // result: diagnostics (and `state`) read by the library's thread when the request finished
   _final(id, status, result) {
     const s = this.pending.get(id);
@@ -471,6 +666,11 @@ class Server {
 }
 
 module.exports = {
+  Device,
+  DeviceBuffer,
+  GPUBufferUsage,
+  GPUMapMode,
+  requestDevice,
   Solver,
   Batch,
   Server,

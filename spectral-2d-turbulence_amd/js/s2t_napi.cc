@@ -381,6 +381,7 @@ napi_value Sync(napi_env env, napi_callback_info info) {
 struct Job {
   SolverBox* box = nullptr;
   napi_ref ref = nullptr;
+  bool in_turn = true;  // false: runs beside the solver's calls (it touches nothing they use)
   uint64_t turn = 0;
   int rc = 0;
   std::string err;
@@ -390,10 +391,10 @@ struct Job {
   virtual int run(s2t_solver* h) = 0;
   virtual napi_value result(napi_env env) = 0;
   void execute() {
-    box->begin(turn);
+    if (in_turn) box->begin(turn);
     rc = run(box->h);
     if (rc != S2T_OK) err = s2t_last_error();  // thread-local: read on the same thread
-    box->end();
+    if (in_turn) box->end();
   }
 };
 
@@ -429,9 +430,9 @@ napi_value queue_job(napi_env env, napi_value v, Job* j, const char* what) {
             napi_create_async_work(env, nullptr, name, job_execute, job_complete, j, &j->work) ==
                 napi_ok;
   if (ok) {
-    j->turn = j->box->ticket();
+    if (j->in_turn) j->turn = j->box->ticket();
     ok = napi_queue_async_work(env, j->work) == napi_ok;
-    if (!ok) {
+    if (!ok && j->in_turn) {
       // give the turn back in order
       j->box->begin(j->turn);
       j->box->end();
@@ -500,6 +501,206 @@ napi_value DiagnosticsAsync(napi_env env, napi_callback_info info) {
   DiagJob* j = new DiagJob();
   j->n = (int)num(env, a.argv[1], 0);
   return queue_job(env, a.argv[0], j, "s2t_diagnostics");
+}
+
+// readDiagnostics of the copied record (s2t_read_copied_diagnostics): waits for that copy only,
+// outside the solver's turn, so the frames encoded after it are not held up (the JS Solver
+// allows no new copy until this read has finished, solver.js:560-608)
+struct CopiedDiagJob : DiagJob {
+  int run(s2t_solver* h) override {
+    spectrum.assign(n / 2 + 1, 0.0);
+    zonal.assign(n, 0.0);
+    return s2t_read_copied_diagnostics(h, &d, spectrum.data(), zonal.data());
+  }
+};
+
+napi_value ReadCopiedDiagnosticsAsync(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  CopiedDiagJob* j = new CopiedDiagJob();
+  j->n = (int)num(env, a.argv[1], 0);
+  j->in_turn = false;
+  return queue_job(env, a.argv[0], j, "s2t_read_copied_diagnostics");
+}
+
+#define SOLVER_CALL(NAME, EXPR)                                  \
+  napi_value NAME(napi_env env, napi_callback_info info) {      \
+    Args a;                                                      \
+    if (!get_args(env, info, a, 1)) return nullptr;              \
+    Held h = handle(env, a.argv[0]);                             \
+    if (!h) return nullptr;                                      \
+    S2T_CALL(env, EXPR);                                         \
+    return nullptr;                                              \
+  }
+
+SOLVER_CALL(Prime, s2t_prime(h))
+SOLVER_CALL(ResetDye, s2t_reset_dye(h))
+SOLVER_CALL(EncodeDiagnostics, s2t_encode_diagnostics(h))
+SOLVER_CALL(CopyDiagnostics, s2t_copy_diagnostics(h))
+
+napi_value Primed(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  int p = 0;
+  S2T_CALL(env, s2t_primed(h, &p));
+  napi_value v;
+  napi_get_boolean(env, p != 0, &v);
+  return v;
+}
+
+// dyeStep(h, dt): dt <= 0 takes the last step's dt on the device (encodeDye)
+napi_value DyeStep(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  S2T_CALL(env, s2t_dye_step(h, num(env, a.argv[1], 0.0)));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------------------------
+// device buffers (s2t_buffer_*) for the GPUBuffer-like objects of the JS device
+// ---------------------------------------------------------------------------------------------
+struct BufBox {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+void finalize_buffer(napi_env, void* data, void*) {
+  BufBox* b = static_cast<BufBox*>(data);
+  s2t_buffer_destroy(b->p);
+  delete b;
+}
+
+BufBox* buffer_box(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !static_cast<BufBox*>(p)->p) {
+    napi_throw_type_error(env, "S2T_HANDLE", "not a device buffer (destroyed?)");
+    return nullptr;
+  }
+  return static_cast<BufBox*>(p);
+}
+
+bool in_range(napi_env env, const BufBox* b, double off, double bytes) {
+  if (off < 0 || bytes < 0 || off + bytes > (double)b->bytes) {
+    napi_throw_range_error(env, "S2T_ARGS", "range outside the buffer");
+    return false;
+  }
+  return true;
+}
+
+napi_value BufferCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  const double bytes = num(env, a.argv[0], 0);
+  BufBox* b = new BufBox();
+  b->bytes = (size_t)bytes;
+  const int rc = s2t_buffer_create(b->bytes, &b->p);
+  if (rc != S2T_OK) {
+    delete b;
+    return throw_status(env, rc);
+  }
+  napi_value ext;
+  if (napi_create_external(env, b, finalize_buffer, nullptr, &ext) != napi_ok) {
+    s2t_buffer_destroy(b->p);
+    delete b;
+    napi_throw_error(env, "S2T_NAPI", "could not wrap the buffer");
+    return nullptr;
+  }
+  return ext;
+}
+
+napi_value BufferDestroy(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 1)) return nullptr;
+  void* p = nullptr;
+  if (napi_get_value_external(env, a.argv[0], &p) == napi_ok && p) {
+    BufBox* b = static_cast<BufBox*>(p);
+    s2t_buffer_destroy(b->p);
+    b->p = nullptr;
+  }
+  return nullptr;
+}
+
+// bufferWrite(buf, offset, typedArray)
+napi_value BufferWrite(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  BufBox* b = buffer_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  bool is = false;
+  napi_is_typedarray(env, a.argv[2], &is);
+  if (!is) {
+    napi_throw_type_error(env, "S2T_ARGS", "expected a typed array");
+    return nullptr;
+  }
+  napi_typedarray_type t;
+  size_t len = 0, aoff = 0;
+  void* data = nullptr;
+  napi_value ab;
+  napi_get_typedarray_info(env, a.argv[2], &t, &len, &data, &ab, &aoff);
+  size_t elem = 1;
+  switch (t) {
+    case napi_int16_array: case napi_uint16_array: elem = 2; break;
+    case napi_int32_array: case napi_uint32_array: case napi_float32_array: elem = 4; break;
+    case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: elem = 8; break;
+    default: elem = 1;
+  }
+  const double off = num(env, a.argv[1], 0);
+  if (!in_range(env, b, off, (double)(len * elem))) return nullptr;
+  S2T_CALL(env, s2t_buffer_write(b->p, (size_t)off, data, len * elem));
+  return nullptr;
+}
+
+// bufferRead(buf, offset, bytes) -> ArrayBuffer
+napi_value BufferRead(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  BufBox* b = buffer_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  const double off = num(env, a.argv[1], 0), bytes = num(env, a.argv[2], 0);
+  if (!in_range(env, b, off, bytes)) return nullptr;
+  void* data = nullptr;
+  napi_value ab;
+  NAPI_CALL(env, napi_create_arraybuffer(env, (size_t)bytes, &data, &ab));
+  S2T_CALL(env, s2t_buffer_read(b->p, (size_t)off, data, (size_t)bytes));
+  return ab;
+}
+
+// bufferCopy(src, srcOffset, dst, dstOffset, bytes)
+napi_value BufferCopy(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 5)) return nullptr;
+  BufBox* src = buffer_box(env, a.argv[0]);
+  if (!src) return nullptr;
+  BufBox* dst = buffer_box(env, a.argv[2]);
+  if (!dst) return nullptr;
+  const double so = num(env, a.argv[1], 0), dof = num(env, a.argv[3], 0),
+               bytes = num(env, a.argv[4], 0);
+  if (!in_range(env, src, so, bytes) || !in_range(env, dst, dof, bytes)) return nullptr;
+  S2T_CALL(env, s2t_buffer_copy(dst->p, (size_t)dof, src->p, (size_t)so, (size_t)bytes));
+  return nullptr;
+}
+
+// fft2Device(h, buf, n, inverse): in place on the solver's stream
+napi_value Fft2Device(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 4)) return nullptr;
+  BufBox* b = buffer_box(env, a.argv[1]);
+  if (!b) return nullptr;
+  const double n = num(env, a.argv[2], 0);
+  if (!(n > 0) || (double)b->bytes < 8.0 * n * n) {
+    napi_throw_range_error(env, "S2T_ARGS", "buffer smaller than an n x n complex field");
+    return nullptr;
+  }
+  Held h = handle(env, a.argv[0]);
+  if (!h) return nullptr;
+  bool inv = false;
+  napi_get_value_bool(env, a.argv[3], &inv);
+  S2T_CALL(env, s2t_fft2_device(h, static_cast<float*>(b->p), inv ? 1 : 0));
+  return nullptr;
 }
 
 // `count` steps (+ dye), then a wait for the device: the Promise settles when they are done
@@ -1114,6 +1315,15 @@ napi_value ServerRef(napi_env env, napi_callback_info info) {
 
 // serverDestroy(h): cancels what is left (those callbacks still arrive), stops the worker, and
 // releases the thread-safe function so the event loop may end
+napi_value ServerInjectFailure(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  ServerBox* b = server_box(env, a.argv[0]);
+  if (!b) return nullptr;
+  S2T_CALL(env, s2t_server_inject_failure(b->sv, (int)num(env, a.argv[1], -1)));
+  return nullptr;
+}
+
 napi_value ServerDestroy(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
@@ -1149,7 +1359,15 @@ napi_value Init(napi_env env, napi_value exports) {
              {"batchUseGraphs", BatchUseGraphs}, {"batchStep", BatchStep},
              {"batchSync", BatchSync}, {"batchAutotune", BatchAutotune},
              {"batchColumnWidth", BatchColumnWidth}, {"autotune", Autotune},
-             {"setClock", SetClock}};
+             {"setClock", SetClock},    {"prime", Prime},
+             {"primed", Primed},        {"resetDye", ResetDye},
+             {"dyeStep", DyeStep},      {"encodeDiagnostics", EncodeDiagnostics},
+             {"copyDiagnostics", CopyDiagnostics},
+             {"readCopiedDiagnosticsAsync", ReadCopiedDiagnosticsAsync},
+             {"bufferCreate", BufferCreate}, {"bufferDestroy", BufferDestroy},
+             {"bufferWrite", BufferWrite}, {"bufferRead", BufferRead},
+             {"bufferCopy", BufferCopy}, {"fft2Device", Fft2Device},
+             {"serverInjectFailure", ServerInjectFailure}};
   for (const Fn& f : fns) {
     napi_value fn;
     NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

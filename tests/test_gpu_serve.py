@@ -118,9 +118,9 @@ def test_cancel_and_completion_callbacks(s2t):
         assert np.array_equal(got, s.read_state())
 
 
-def test_worker_failure_fails_every_outstanding_request(s2t, monkeypatch):
-    """If the worker stops on an error (forced from the host: S2T_SERVER_FAIL_AFTER = 1 makes the
-    second admission fail), every request still queued or running fails and is called back once,
+def test_worker_failure_fails_every_outstanding_request(s2t):
+    """If the worker stops on an error (forced from the host: s2t_server_inject_failure(1) makes
+    the second admission fail), every request still queued or running fails and is called back once,
     so no caller waits forever; the server then accepts nothing more and reports the error."""
     import threading
     seen, lock, done = {}, threading.Lock(), threading.Event()
@@ -132,10 +132,9 @@ def test_worker_failure_fails_every_outstanding_request(s2t, monkeypatch):
             if len(seen) == 3:
                 done.set()
 
-    monkeypatch.setenv("S2T_SERVER_FAIL_AFTER", "1")
     sv = s2t.Server({64: 1})
-    monkeypatch.delenv("S2T_SERVER_FAIL_AFTER")
     try:
+        sv.inject_failure(1)
         sv.set_callback(on_final)
         ids = [sv.submit(n=64, steps=20, seed=k, **PHYS) for k in range(3)]
         assert done.wait(60)

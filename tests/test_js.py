@@ -49,8 +49,11 @@ def test_js_scenarios_match_oracle():
 
 @pytest.mark.gpu
 def test_js_harness_fixtures():
-    r = subprocess.run([NODE, os.path.join(JS, "harness.js")], capture_output=True, text=True,
-                       timeout=300)
+    """The checks of the reference's browser harness (tests/index.html:114-207) through the JS
+    surface: buffers, a compute pass, queue.submit, copyBufferToBuffer and mapAsync for the FFTs and
+    the steps; encodeDiagnostics / copyDiagnostics / readDiagnostics for the record."""
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_harness.js")], capture_output=True,
+                       text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL CHECKS PASS" in r.stdout
@@ -133,10 +136,10 @@ const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e
   const b = await m.Solver.create(null, { n: 128, dyeScale: 1 });
   for (const s of [a, b]) { s.configure(phys); s.seed({ k0: 8, amplitude: 5, seed: 3 }); }
   // calls take effect in program order whichever thread runs them
-  const d0 = a.readDiagnostics();
+  const d0 = a.diagnosticsAsync();
   a.step(2);
   const st = a.stepAsync(3);
-  const d5 = a.readDiagnostics();
+  const d5 = a.diagnosticsAsync();
   a.step(1);
   const [r0, r5] = await Promise.all([d0, d5]);
   await st;
@@ -152,7 +155,7 @@ const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e
 
 @pytest.mark.gpu
 def test_js_async_calls_keep_program_order():
-    """stepAsync and readDiagnostics run on worker threads, but every call on a solver takes
+    """stepAsync and diagnosticsAsync run on worker threads, but every call on a solver takes
     effect in the order the program made it (a diagnostics read issued before some steps sees
     the state before them), and stepAsync steps are the synchronous steps, bit for bit."""
     import json
@@ -225,6 +228,7 @@ const m = require('./index.js');
 const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };
 (async () => {
   const sv = new m.Server([{ n: 64, slots: 1 }]);
+  sv.injectFailure(1);
   const reqs = [0, 1, 2].map((k) => sv.submit({ n: 64, steps: 20, seed: k }, phys));
   const out = await Promise.all(reqs.map((r) => r.result.then(
     (d) => ({ status: d.status }), (e) => ({ status: e.status, message: e.message }))));
@@ -236,14 +240,87 @@ const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e
 
 @pytest.mark.gpu
 def test_js_server_worker_failure_settles_every_promise():
-    """A worker failure (forced from the host with S2T_SERVER_FAIL_AFTER = 1: the second admission
+    """A worker failure (forced from the host with injectFailure(1): the second admission
     fails) rejects every outstanding Promise with REQ.FAILED and the error text, and the process
     ends by itself."""
     import json
-    env = dict(os.environ, S2T_SERVER_FAIL_AFTER="1")
     r = subprocess.run([NODE, "-e", FAIL_JS], cwd=JS, capture_output=True, text=True,
-                       timeout=120, env=env)
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = json.loads(r.stdout)
     assert [o["status"] for o in out] == [2, 5, 5]
     assert all("injected" in o["message"] for o in out[1:])
+
+
+FRAMES_JS = r"""
+const fs = require('fs');
+const { runFrames } = require('./headless.js');
+const m = require('./index.js');
+(async () => {
+  const n = 128, frames = 14, substeps = 3;
+  const config = { substeps, viscosity: 0.3, hyperP: 4, beta: 0, drag: 0, forceAmp: 0, cfl: 0.5,
+                   dtMin: 1e-5, dtMax: 0.02, dyeFade: 0.999 };
+  const { solver, records } = await runFrames({ n, dyeScale: 2, frames, config,
+                                                scenario: 'dipole' });
+  const st = solver.readState(), dye = solver.currentDye;
+  const simTime = solver.simTime, d = solver.diagnostics();
+  // the same run as plain steps with dye on a solver seeded the same way
+  const p = await m.Solver.create(null, { n, dyeScale: 2 });
+  p.configure({ ...m.DEFAULT_CONFIG, ...m.SCENARIOS.dipole.config, ...config });
+  p.reset({ field: m.SCENARIOS.dipole.build(n, config) });
+  const want = [];
+  let lastState = null;
+  for (const r of records) {
+    p.step(substeps * (r.frame + 1) - p.diagnostics().stepIndex, { dye: true });
+    const q = p.diagnostics();
+    want.push({ energy: q.energy, enstrophy: q.enstrophy, stepIndex: q.stepIndex, time: q.time });
+    lastState = p.readState();
+  }
+  p.step(substeps * frames - p.diagnostics().stepIndex, { dye: true });
+  const ps = p.readState(), pd = p.currentDye;
+  let same = st.length === ps.length, sameDye = dye.length === pd.length;
+  for (let i = 0; same && i < st.length; i++) same = st[i] === ps[i];
+  for (let i = 0; sameDye && i < dye.length; i++) sameDye = dye[i] === pd[i];
+  fs.writeFileSync(process.argv[1], Buffer.from(lastState.buffer));
+  console.log(JSON.stringify({ records: records.map((r) => ({ frame: r.frame, stepIndex: r.stepIndex,
+    energy: r.energy, enstrophy: r.enstrophy, time: r.time, dt: r.dt })), want, same, sameDye,
+    stepIndex: d.stepIndex, time: d.time, simTime, primed: solver.primed }));
+  solver.destroy();
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_headless_frame_loop(s2t, tmp_path):
+    """The main.js frame loop run headless (js/headless.js): prime, substeps x [encodeStep,
+    encodeDye, encodeReduce], diagnostics every 6th frame copied and read asynchronously. The final
+    state and dye equal solver.step(frames * substeps, {dye: true}) bit for bit; each diagnostics
+    record equals the plain solver's at the same step and matches the FP64 oracle's E and Z of the
+    final state; simTime follows main.js:445 (the last record's dt times the substeps, per frame
+    after the first record)."""
+    import json
+
+    import numpy as np
+
+    from oracle.spectral import Scheme
+    state = tmp_path / "state.f32"
+    r = subprocess.run([NODE, "-e", FRAMES_JS, str(state)], cwd=JS, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["same"] and out["sameDye"] and out["primed"]
+    assert out["stepIndex"] == 14 * 3
+    assert [rec["frame"] for rec in out["records"]] == [0, 6, 12]
+    for rec, want in zip(out["records"], out["want"]):
+        assert rec["stepIndex"] == want["stepIndex"] == 3 * (rec["frame"] + 1)
+        assert rec["energy"] == want["energy"] and rec["enstrophy"] == want["enstrophy"]
+        assert rec["time"] == want["time"]
+    # the oracle's FP64 E and Z of the state each record was taken from (the plain solver's state
+    # at the last record's step, equal to the driver's by the checks above)
+    w = s2t.from_interleaved(np.fromfile(state, np.float32), 128)
+    o = Scheme(128)
+    last = out["records"][-1]
+    assert last["energy"] == pytest.approx(o.energy(w), rel=1e-12)
+    assert last["enstrophy"] == pytest.approx(o.enstrophy(w), rel=1e-12)
+    # simTime advances by the last received record's dt per frame (main.js:445)
+    assert 0 < out["simTime"] <= 14 * 3 * max(r["dt"] for r in out["records"]) * (1 + 1e-12)

@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 B_CANONICAL = 208.0    # bytes/pt/step of the fused full-complex schedule used for the target
 PROFILE_EVERY = 4      # the dominant kernel is timed in one step of this many
+SETTLE_S = 1.5         # seconds of untimed stepping after the warm-up, before the timed steps
 
 
 def measured_traffic(kernel, n):
@@ -118,7 +119,7 @@ def ensemble_params(i, n):
                 force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
 
 
-def bench_ensemble(s2t, steps, warmup, plain_steps):
+def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
     """64 independent members (32 x 256^2, 24 x 512^2, 8 x 1024^2) advanced together by the
     batch path (one launch per kernel and size class, hipGraph replay), against a loop over the
     plain single-instance path on the same GPU."""
@@ -138,13 +139,16 @@ def bench_ensemble(s2t, steps, warmup, plain_steps):
         b.step(steps)
         b.sync()
         el_default = time.perf_counter() - t0
-        widths = b.autotune(steps=20)
-        b.step(warmup)
-        b.sync()
-        t0 = time.perf_counter()
-        b.step(steps)
-        b.sync()
-        el = time.perf_counter() - t0
+        if tune:
+            widths = b.autotune(steps=20)
+            b.step(warmup)
+            b.sync()
+            t0 = time.perf_counter()
+            b.step(steps)
+            b.sync()
+            el = time.perf_counter() - t0
+        else:
+            widths, el = default_widths, el_default
         bad = [i for i in range(len(sizes)) if not np.isfinite(b.diagnostics(i)["energy"])]
     if bad:
         raise RuntimeError("non-finite ensemble members %s" % bad)
@@ -197,40 +201,112 @@ def serve_requests(count, seed):
     return out
 
 
-def bench_serve(s2t, rates, count, seed=5):
-    """Poisson arrivals at each rate (requests/s) into the continuous-batching server; reports
-    completion-latency percentiles and the aggregate completed Mpts·steps/s."""
-    phys = dict(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5, dt_max=0.02)
-    reqs = serve_requests(count, seed)
-    work = float(np.mean([r["n"] ** 2 * r["steps"] for r in reqs]))
-    lines = []
-    for rate in rates:
-        rng = np.random.default_rng(seed + 1)
-        gaps = rng.exponential(1.0 / rate, size=count)
-        with s2t.Server(SERVE_SLOTS, budget_bytes=200e9) as sv:
-            t0 = time.perf_counter()
-            due = t0
-            for r, g in zip(reqs, gaps):
-                due += g
+SERVE_PHYS = dict(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5,
+                  dt_max=0.02)
+
+
+def run_served(s2t, reqs, gaps):
+    """Submit `reqs` at the given gaps (s; None: all at once) into a server with a 200 GB budget
+    and wait for them; returns the server's statistics and the makespan."""
+    with s2t.Server(SERVE_SLOTS, budget_bytes=200e9) as sv:
+        t0 = time.perf_counter()
+        due = t0
+        for k, r in enumerate(reqs):
+            if gaps is not None:
+                due += gaps[k]
                 delay = due - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                sv.submit(**r, **phys)
-            sv.drain(600)
-            st = sv.stats()
-        lines.append({"rate_req_s": rate, "requests": count, "completed": st["completed"],
+            sv.submit(**r, **SERVE_PHYS)
+        sv.drain(900)
+        span = time.perf_counter() - t0
+        st = sv.stats()
+    return st, span
+
+
+def run_plain_loop(s2t, reqs):
+    """The same requests one after another on plain solvers of their sizes (seeding included,
+    as the server's admission does): seconds in total."""
+    total = 0.0
+    solvers = {}
+    try:
+        for r in reqs:
+            n = r["n"]
+            if n not in solvers:
+                solvers[n] = s2t.Solver(n)
+            s = solvers[n]
+            t0 = time.perf_counter()
+            s.configure(**SERVE_PHYS)
+            if r["scenario"] == "turbulence":
+                s.seed_spectrum(k0=14.0, amplitude=6.0, seed=r["seed"])
+            else:
+                s.seed_physical(serve_field(r["scenario"], n, r["scenario_param"]))
+            s.step(r["steps"])
+            s.sync()
+            total += time.perf_counter() - t0
+    finally:
+        for s in solvers.values():
+            s.close()
+    return total
+
+
+def serve_field(scenario, n, param):
+    """Physical vorticity of a server scenario on x_j = 2 pi j / N (src/scenarios.js:82-145)."""
+    h = 2 * np.pi / n
+    y, x = np.meshgrid(np.arange(n) * h, np.arange(n) * h, indexing="ij")
+    if scenario == "taylor-green":
+        return (6 * np.cos(2 * x) * np.cos(2 * y)).astype(np.float32)
+    if scenario == "dipole":
+        a = 0.3
+        amp = 12 / (np.pi * a * a)
+        cores = [(0.6 * np.pi, np.pi - 0.45, a, amp), (0.6 * np.pi, np.pi + 0.45, a, -amp)]
+    else:
+        d = param if param > 0 else 0.7
+        cores = [(np.pi - d / 2, np.pi, 0.22, 30.0), (np.pi + d / 2, np.pi, 0.22, 30.0)]
+    f = np.zeros((n, n))
+    for cx, cy, a, amp in cores:
+        dx = (x - cx + np.pi) % (2 * np.pi) - np.pi
+        dy = (y - cy + np.pi) % (2 * np.pi) - np.pi
+        r2 = dx * dx + dy * dy
+        f += np.where(r2 < 25 * a * a, amp * np.exp(-r2 / (a * a)), 0.0)
+    return f.astype(np.float32)
+
+
+def bench_serve(s2t, rates, count, seed=5, plain=True):
+    """Poisson arrivals at each rate (requests/s) into the continuous-batching server, then all at
+    once (the saturation throughput); completion-latency percentiles and completed Mpts*steps/s
+    per rate, and the same requests run one at a time on plain solvers."""
+    reqs = serve_requests(count, seed)
+    work = float(np.sum([r["n"] ** 2 * r["steps"] for r in reqs]))
+    lines = []
+    for rate in list(rates) + [None]:
+        gaps = None if rate is None else np.random.default_rng(seed + 1).exponential(1.0 / rate,
+                                                                                    size=count)
+        st, span = run_served(s2t, reqs, gaps)
+        lines.append({"rate_req_s": rate if rate is not None else "all at once",
+                      "offered_mpts_steps_per_s": round(rate * work / count / 1e6, 1)
+                      if rate is not None else None,
+                      "requests": count, "completed": st["completed"],
                       "rejected": st["rejected"], "p50_latency_s": round(st["p50_latency_s"], 4),
                       "p99_latency_s": round(st["p99_latency_s"], 4),
                       "mean_latency_s": round(st["mean_latency_s"], 4),
-                      "mpts_steps_per_s": round(st["mpts_steps_per_s"], 1)})
+                      "mpts_steps_per_s": round(st["mpts_steps_per_s"], 1),
+                      "makespan_s": round(span, 3), "peak_gb": round(st["peak_bytes"] / 1e9, 3)})
         print(json.dumps(lines[-1]), file=sys.stderr, flush=True)
-    return {"metric": "serving Mpts·RK3-steps/s under Poisson arrivals",
-            "value": max(l["mpts_steps_per_s"] for l in lines), "unit": "Mpts·steps/s",
-            "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic: scenario mix, N in {128..2048}^2, steps log-uniform 100..5000",
-            "config": {"workload": "continuous-batching server", "slots": SERVE_SLOTS,
-                       "mean_work_pts_steps": work, "budget_bytes": 200e9},
-            "sweep": lines}
+    saturation = lines[-1]["mpts_steps_per_s"]
+    out = {"metric": "serving Mpts·RK3-steps/s under Poisson arrivals",
+           "value": saturation, "unit": "Mpts·steps/s",
+           "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: scenario mix, N in {128..2048}^2, steps log-uniform 100..5000",
+           "config": {"workload": "continuous-batching server", "slots_per_chunk": SERVE_SLOTS,
+                      "requests": count, "work_pts_steps": work, "budget_bytes": 200e9},
+           "saturation_mpts_steps_per_s": saturation, "sweep": lines}
+    if plain:
+        el = run_plain_loop(s2t, reqs)
+        out["plain_loop"] = {"seconds": round(el, 3),
+                             "mpts_steps_per_s": round(work / el / 1e6, 1)}
+        out["served_vs_plain"] = round(saturation / (work / el / 1e6), 3)
+    return out
 
 
 def blob_dye(m, seed, count=12):
@@ -435,7 +511,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="step: leave out the ensemble and serving summaries")
     ap.add_argument("--workload", choices=["step", "ensemble", "serve", "decaying", "horizons"],
                     default="step",
                     help="step: one 4096^2 instance (default); ensemble: 64 mixed-size members; "
@@ -472,7 +550,7 @@ def main(argv=None):
         print(json.dumps(bench_decaying(s2t, args.steps, args.warmup)))
         return 0
     if args.workload == "serve":
-        rates = [float(r) for r in args.rates.split(",")] if args.rates else [5.0, 15.0, 30.0]
+        rates = [float(r) for r in args.rates.split(",")] if args.rates else [10.0, 30.0, 60.0]
         print(json.dumps(bench_serve(s2t, rates, args.requests)))
         return 0
     n = args.n
@@ -482,6 +560,14 @@ def main(argv=None):
     solver.seed_spectrum(k0=1.0, amplitude=0.0, seed=1)  # from rest
     solver.step(args.warmup)
     solver.sync()
+    # untimed steps until the GPU clocks have settled (at least SETTLE_S of back-to-back work
+    # after the warm-up): a few warm-up steps alone leave the first timed steps at lower clocks
+    t_settle = time.perf_counter()
+    settle_steps = 0
+    while time.perf_counter() - t_settle < SETTLE_S:
+        solver.step(50)
+        solver.sync()
+        settle_steps += 50
 
     # timed region: HIP events only around the dominant kernel (the column kernel), in one step
     # of four (all three stages of that step); each event pair costs a few microseconds of stream
@@ -543,9 +629,30 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(names[dom], n), "avg_launch_us": round(avg_s * 1e6, 2),
-                     "bytes_per_launch": bytes_launch},
+                     "bytes_per_launch": bytes_launch,
+                     "limiter": "vector-instruction issue and latency at 2 workgroups per CU "
+                                "(LDS-limited), not HBM: 32.7 M VALU instructions per launch are "
+                                "about half its time (profiles/step_kernels_4096_forced_"
+                                "turbulence.md)"},
         "kernels": kernels,
+        "warmup_settle": {"steps": settle_steps, "seconds": SETTLE_S},
     }
+    if not args.no_workloads:
+        # the second half of the metric: batched-ensemble and served throughput, each against a
+        # loop over the plain single-instance path on the same GPU
+        ens = bench_ensemble(s2t, 100, 10, 20, tune=False)
+        out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "plain_loop",
+                                               "speedup_vs_plain_loop")}
+        out["ensemble"]["workload"] = ens["config"]["workload"]
+        srv = bench_serve(s2t, [10.0, 30.0, 60.0], 80)
+        out["serve"] = {"saturation_mpts_steps_per_s": srv["saturation_mpts_steps_per_s"],
+                        "plain_loop": srv["plain_loop"],
+                        "served_vs_plain": srv["served_vs_plain"],
+                        "sweep": [{k: l[k] for k in ("rate_req_s", "p50_latency_s",
+                                                     "p99_latency_s", "mpts_steps_per_s")}
+                                  for l in srv["sweep"]],
+                        "workload": "80 requests, N in {128..2048}^2, steps log-uniform "
+                                    "100..5000, four scenarios, 200 GB budget"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, params, steps=args.cpu_steps)
     print(json.dumps(out))

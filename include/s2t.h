@@ -294,7 +294,7 @@ typedef struct s2t_server s2t_server;
 #define S2T_REQ_FAILED 5 /* the server's worker stopped on an error; it accepts nothing more */
 
 typedef struct s2t_request {
-  int n;                 /* grid size; must be one of the server's classes */
+  int n;                 /* grid size (4^a or 2*4^a, 16..4096) */
   int steps;             /* RK3 steps to run */
   int scenario;          /* S2T_SCENARIO_* */
   int keep_state;        /* nonzero: keep the final spectrum for s2t_server_read_state */
@@ -308,17 +308,23 @@ typedef struct s2t_server_stats {
   uint64_t submitted, completed, rejected, cancelled;
   double p50_latency_s, p99_latency_s, mean_latency_s;
   double mpts_steps_per_s; /* completed work / time from first submit to last completion */
-  double budget_bytes, used_bytes;
+  double budget_bytes, used_bytes; /* the budget (0 = none) and the device bytes held now */
+  double peak_bytes;               /* the most held at once */
 } s2t_server_stats;
 
-/* Classes of slots: sizes[i] with slots[i] concurrent requests each, allocated up front and
- * checked against budget_bytes (0 = no limit). A worker thread admits requests as slots free. */
+/* A server with a device-memory budget (budget_bytes; 0 = no limit). The instances of one grid size
+ * live in chunks of slots allocated as requests need them; sizes[i] (any supported size) gets
+ * chunks of slots[i] instances and one chunk at once (nclasses may be 0; other sizes get chunks
+ * of about 2^22 points, at most 32 instances). A request takes a free slot, or a new chunk if the
+ * budget has room (idle chunks are freed to make room); otherwise it waits, and later requests
+ * that fit are admitted past it. A worker thread admits and steps. */
 int s2t_server_create(const int* sizes, const int* slots, int nclasses, double budget_bytes,
                       s2t_server** out);
 /* Cancels the requests still queued or running (their callbacks run), stops the worker and
  * frees everything. */
 int s2t_server_destroy(s2t_server* s);
-/* Thread-safe. A size without a class is rejected (status S2T_REQ_REJECTED). */
+/* Thread-safe. An unsupported size, or one of which not even a single instance fits in the budget,
+ * is rejected (status S2T_REQ_REJECTED). */
 int s2t_server_submit(s2t_server* s, const s2t_request* r, uint64_t* id);
 /* Status, FP64 diagnostics of the final state and submit-to-completion latency (-1 until done). */
 int s2t_server_poll(s2t_server* s, uint64_t id, int* status, s2t_diag* diag, double* latency_s);

@@ -90,7 +90,7 @@ class ServerStats(ctypes.Structure):
                 ("p50_latency_s", ctypes.c_double),
                 ("p99_latency_s", ctypes.c_double), ("mean_latency_s", ctypes.c_double),
                 ("mpts_steps_per_s", ctypes.c_double), ("budget_bytes", ctypes.c_double),
-                ("used_bytes", ctypes.c_double)]
+                ("used_bytes", ctypes.c_double), ("peak_bytes", ctypes.c_double)]
 
 
 class S2TError(RuntimeError):
@@ -615,7 +615,8 @@ class Batch:
 
 
 class Server:
-    """Continuous-batching server (s2t_server_*): classes of slots per grid size."""
+    """Continuous-batching server (s2t_server_*): chunks of slots per grid size, allocated as
+    requests need them within a device-memory budget; `classes` maps sizes to slots per chunk."""
 
     def __init__(self, classes, budget_bytes=0.0):
         self.lib = load()

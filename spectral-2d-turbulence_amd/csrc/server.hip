@@ -1,13 +1,16 @@
 // Continuous-batching serving loop (C ABI s2t_server_*).
 //
 // Requests (grid size, step count, scenario, parameters) arrive from any thread. A native worker
-// thread admits each into a free slot of the class of its grid size, seeds it on the device and
-// sets its step budget; every active class is then stepped on its own stream (one launch per
-// fused kernel covers all slots of a class; idle slots exit at once). Each slot retires itself
-// on the device after exactly its budgeted steps (the step epilogue), so a result does not depend on
-// when the host polls or on which other requests share the class: it is bit-identical to the same
+// thread admits them against a device-memory budget: the instances of one grid size live in
+// chunks (a Solver of a fixed number of slots: one launch per fused kernel covers every slot of a
+// chunk, idle slots exit at once); a request takes a free slot of a chunk of its size, or the
+// worker allocates a new chunk if the budget has room, freeing idle chunks of other sizes when it
+// does not. A request that fits nowhere yet waits; later requests that fit are admitted past it.
+// Every chunk with active slots is stepped on its own stream. Each slot retires itself on the
+// device after exactly its budgeted steps (the step epilogue), so a result does not depend on
+// when the host polls or on which other requests share the chunk: it is bit-identical to the same
 // request run alone. Completions record their latency and FP64 diagnostics (and, on request, the
-// final spectrum); the server reports p50/p99 latency and aggregate throughput.
+// final spectrum); the server reports p50/p99 latency, aggregate throughput and memory use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -102,10 +105,11 @@ struct Record {
   bool keep_state = false;
 };
 
-struct Class {
+// A chunk: one Solver with `slots` instances of one grid size.
+struct Chunk {
   Solver* s = nullptr;
   std::vector<int64_t> slot_req;  // request id per slot, -1 when free
-  std::vector<int> slot_phase;    // class buffer phase when the slot's request was admitted
+  std::vector<int> slot_phase;    // chunk buffer phase when the slot's request was admitted
   int active = 0;
   double ms_per_step = 1.0;       // running estimate
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -113,7 +117,8 @@ struct Class {
 };
 
 struct Server {
-  std::map<int, Class> cls;       // grid size -> class
+  std::map<int, std::vector<Chunk*>> cls;  // grid size -> chunks (worker thread only)
+  std::map<int, int> chunk_slots;          // slots per chunk of a configured size
   std::mutex mu;
   std::condition_variable cv;
   std::deque<Request> pending;
@@ -121,8 +126,8 @@ struct Server {
   std::thread worker;
   std::atomic<bool> stop{false};
   uint64_t next_id = 1;
-  double target_ms = 4.0;         // GPU time per class per poll
-  size_t budget = 0, used = 0;
+  double target_ms = 4.0;         // GPU time per chunk per poll
+  size_t budget = 0, used = 0, peak = 0;  // device bytes: limit (0 = none), in chunks now, most
   // statistics
   std::vector<double> latencies;
   double work_pts_steps = 0;      // sum n^2 * steps of completed requests
@@ -149,17 +154,63 @@ static void notify(Server* sv, uint64_t id, int status) {
   if (cb) cb(id, status, user);
 }
 
-static size_t class_bytes(int n, int slots) {
-  // device bytes of a class of `slots` instances of size n (solver_new's allocations)
+static size_t chunk_bytes(int n, int slots) {
+  // device bytes of a chunk of `slots` instances of size n (solver_new's allocations, plus the
+  // full-field scratch that seeding from a physical field allocates)
   const int K = n / 3, KC = K + 1, NJ = 2 * K + 1;
   const size_t pitch = (NJ + 63) / 64 * 64;
   const size_t kp = (KC + 15) / 16 * 16;
   const size_t state = (size_t)KC * pitch * 8, plane = (size_t)n * kp * 8;
-  // per slot: 3 states, 5 mixed planes; per class: 3 full fields and one state of hook scratch
+  // per slot: 3 states, 5 mixed planes; per chunk: 3 full fields and one state of scratch
   return (size_t)slots * (3 * state + 5 * plane) + (size_t)n * n * 3 * 8 + state;
 }
 
-static void admit(Server* sv, Class& c, int slot, const Request& r) {
+// Slots per chunk of a size without a configured class: enough instances that one launch covers
+// about 2^22 points (a chunk of small grids fills the GPU), at most 32, at least 1.
+static int default_chunk_slots(int n) {
+  const long want = (1L << 22) / ((long)n * n);
+  return (int)std::max(1L, std::min(32L, want));
+}
+
+static int slots_for(const Server* sv, int n) {
+  auto it = sv->chunk_slots.find(n);
+  return it != sv->chunk_slots.end() ? it->second : default_chunk_slots(n);
+}
+
+static Chunk* new_chunk(int n, int slots) {
+  Chunk* c = new Chunk();
+  try {
+    c->s = solver_new(n, 1, slots, false);
+    c->slot_req.assign(slots, -1);
+    c->slot_phase.assign(slots, 0);
+    c->bytes = chunk_bytes(n, slots);
+    hipchk(hipEventCreate(&c->e0), "hipEventCreate");
+    hipchk(hipEventCreate(&c->e1), "hipEventCreate");
+    // a new chunk starts with every slot idle
+    for (int k = 0; k < slots; ++k) {
+      const int zero = 0;
+      hipchk(hipMemcpy(&c->s->inst[k].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
+             "hipMemcpy");
+    }
+  } catch (...) {
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    solver_free(c->s);
+    delete c;
+    throw;
+  }
+  return c;
+}
+
+static void free_chunk(Chunk* c) {
+  if (!c) return;
+  if (c->e0) (void)hipEventDestroy(c->e0);
+  if (c->e1) (void)hipEventDestroy(c->e1);
+  solver_free(c->s);
+  delete c;
+}
+
+static void admit(Server* sv, Chunk& c, int slot, const Request& r) {
   Solver* s = c.s;
   const s2t_request& q = r.req;
   configure_slot(s, slot, q.params);
@@ -181,7 +232,47 @@ static void admit(Server* sv, Class& c, int slot, const Request& r) {
   rc.t_start = now_s();
 }
 
-static void complete(Server* sv, Class& c, int slot) {
+// A chunk of size n with a free slot for a new request: an existing one, else a new one if the
+// budget has room (freeing idle chunks of any size first when it does not); null when neither.
+// Worker thread only.
+static Chunk* place(Server* sv, int n) {
+  for (Chunk* c : sv->cls[n])
+    if (c->active < (int)c->slot_req.size()) return c;
+  const int want = slots_for(sv, n);
+  auto fits = [&](size_t bytes) { return sv->budget == 0 || sv->used + bytes <= sv->budget; };
+  size_t bytes = chunk_bytes(n, want);
+  int slots = want;
+  if (!fits(bytes)) {
+    // free idle chunks (no active slot) until the new chunk fits, largest first
+    std::vector<std::pair<size_t, std::pair<int, Chunk*>>> idle;
+    for (auto& kv : sv->cls)
+      for (Chunk* c : kv.second)
+        if (c->active == 0) idle.push_back({c->bytes, {kv.first, c}});
+    std::sort(idle.begin(), idle.end(),
+              [](const auto& a, const auto& b) { return a.first > b.first; });
+    for (auto& it : idle) {
+      if (fits(bytes)) break;
+      auto& v = sv->cls[it.second.first];
+      v.erase(std::find(v.begin(), v.end(), it.second.second));
+      free_chunk(it.second.second);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->used -= it.first;
+    }
+    if (!fits(bytes) && fits(chunk_bytes(n, 1))) {
+      slots = 1;  // a one-slot chunk where the full one does not fit
+      bytes = chunk_bytes(n, 1);
+    }
+    if (!fits(bytes)) return nullptr;
+  }
+  Chunk* c = new_chunk(n, slots);
+  sv->cls[n].push_back(c);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->used += c->bytes;
+  sv->peak = std::max(sv->peak, sv->used);
+  return c;
+}
+
+static void complete(Server* sv, Chunk& c, int slot) {
   const uint64_t id = (uint64_t)c.slot_req[slot];
   bool keep;
   int steps;
@@ -190,7 +281,7 @@ static void complete(Server* sv, Class& c, int slot) {
     keep = sv->rec[id].keep_state;
     steps = sv->rec[id].steps;
   }
-  // the class kept rotating its buffers after this slot retired: its final state is where its
+  // the chunk kept rotating its buffers after this slot retired: its final state is where its
   // own last step left it
   const int phase = (c.slot_phase[slot] + steps) % 3;
   s2t_diag d{};
@@ -216,8 +307,8 @@ static void complete(Server* sv, Class& c, int slot) {
 }
 
 // Stop a running request whose cancellation was asked for: its slot goes idle on the device
-// (no step kernel of the class is in flight when the worker calls this) and is freed.
-static void cancel_slot(Server* sv, Class& c, int slot) {
+// (no step kernel of the chunk is in flight when the worker calls this) and is freed.
+static void cancel_slot(Server* sv, Chunk& c, int slot) {
   const uint64_t id = (uint64_t)c.slot_req[slot];
   const int zero = 0;
   hipchk(hipMemcpy(&c.s->inst[slot].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
@@ -234,8 +325,9 @@ static void cancel_slot(Server* sv, Class& c, int slot) {
 static void worker_loop(Server* sv) {
   try {
     while (true) {
-      // 1. admission: FIFO per grid size, into free slots
-      std::vector<std::pair<int, Request>> take;
+      // 1. admission, FIFO: each pending request takes a slot wherever `place` finds one; one that
+      // fits nowhere yet keeps its place in the queue
+      std::vector<Request> queue;
       std::vector<uint64_t> dropped;  // queued requests cancelled by the shutdown
       {
         std::unique_lock<std::mutex> lk(sv->mu);
@@ -250,11 +342,13 @@ static void worker_loop(Server* sv) {
           }
           sv->pending.clear();
           for (auto& kv : sv->cls)
-            for (int64_t id : kv.second.slot_req)
-              if (id >= 0) sv->rec[(uint64_t)id].cancel = true;
+            for (Chunk* c : kv.second)
+              for (int64_t id : c->slot_req)
+                if (id >= 0) sv->rec[(uint64_t)id].cancel = true;
         }
         bool any_active = false;
-        for (auto& kv : sv->cls) any_active |= kv.second.active > 0;
+        for (auto& kv : sv->cls)
+          for (Chunk* c : kv.second) any_active |= c->active > 0;
         if (!any_active && sv->pending.empty()) {
           if (sv->stop) {
             lk.unlock();
@@ -264,66 +358,66 @@ static void worker_loop(Server* sv) {
           sv->cv.wait_for(lk, std::chrono::milliseconds(2));
           continue;
         }
-        std::map<int, int> free_slots;
-        for (auto& kv : sv->cls) {
-          int f = 0;
-          for (int64_t r : kv.second.slot_req) f += r < 0;
-          free_slots[kv.first] = f;
-        }
-        for (auto it = sv->pending.begin(); it != sv->pending.end();) {
-          auto fs = free_slots.find(it->req.n);
-          if (fs != free_slots.end() && fs->second > 0) {
-            fs->second -= 1;
-            take.push_back({it->req.n, *it});
-            it = sv->pending.erase(it);
-          } else {
-            ++it;
-          }
-        }
+        queue.assign(sv->pending.begin(), sv->pending.end());
       }
       for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
-      for (auto& tr : take) {
-        if (sv->fail_after >= 0 && sv->admitted >= sv->fail_after)
+      std::vector<uint64_t> taken;
+      for (const Request& r : queue) {
+        {
+          // cancelled while queued: s2t_server_cancel removed it from `pending`
+          std::lock_guard<std::mutex> lk(sv->mu);
+          if (sv->rec[r.id].status != S2T_REQ_QUEUED) continue;
+        }
+        Chunk* c = place(sv, r.req.n);
+        if (!c) continue;
+        const int fa = sv->fail_after.load();
+        if (fa >= 0 && sv->admitted >= fa)
           throw Fail(S2T_EHIP, "worker failure injected by s2t_server_inject_failure");
-        Class& c = sv->cls[tr.first];
         int slot = 0;
-        while (c.slot_req[slot] >= 0) ++slot;
-        admit(sv, c, slot, tr.second);
+        while (c->slot_req[slot] >= 0) ++slot;
+        {
+          std::lock_guard<std::mutex> lk(sv->mu);
+          for (auto p = sv->pending.begin(); p != sv->pending.end(); ++p)
+            if (p->id == r.id) {
+              sv->pending.erase(p);
+              break;
+            }
+        }
+        admit(sv, *c, slot, r);
         sv->admitted += 1;
       }
-      // cancellations of running requests, between steps (every class stream is idle here)
+      // cancellations of running requests, between steps (every chunk stream is idle here)
       std::vector<uint64_t> stopped;
-      for (auto& kv : sv->cls) {
-        Class& c = kv.second;
-        for (int slot = 0; slot < (int)c.slot_req.size(); ++slot) {
-          if (c.slot_req[slot] < 0) continue;
-          bool want;
-          {
-            std::lock_guard<std::mutex> lk(sv->mu);
-            want = sv->rec[(uint64_t)c.slot_req[slot]].cancel;
+      for (auto& kv : sv->cls)
+        for (Chunk* c : kv.second)
+          for (int slot = 0; slot < (int)c->slot_req.size(); ++slot) {
+            if (c->slot_req[slot] < 0) continue;
+            bool want;
+            {
+              std::lock_guard<std::mutex> lk(sv->mu);
+              want = sv->rec[(uint64_t)c->slot_req[slot]].cancel;
+            }
+            if (want) {
+              stopped.push_back((uint64_t)c->slot_req[slot]);
+              cancel_slot(sv, *c, slot);
+            }
           }
-          if (want) {
-            stopped.push_back((uint64_t)c.slot_req[slot]);
-            cancel_slot(sv, c, slot);
-          }
-        }
-      }
       for (uint64_t id : stopped) notify(sv, id, S2T_REQ_CANCELLED);
-      // 2. step every class with active members, each on its own stream
-      std::vector<std::pair<Class*, int>> ran;
-      for (auto& kv : sv->cls) {
-        Class& c = kv.second;
-        if (c.active == 0) continue;
-        const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c.ms_per_step)));
-        ensure_primed(c.s);
-        hipchk(hipEventRecord(c.e0, c.s->stream), "hipEventRecord");
-        for (int i = 0; i < k; ++i) step_once(c.s, false);
-        hipchk(hipEventRecord(c.e1, c.s->stream), "hipEventRecord");
-        ran.push_back({&c, k});
-      }
-      // 3. poll: per-slot device flags tell which members retired
+      // 2. step every chunk with active slots, each on its own stream
+      std::vector<std::pair<Chunk*, int>> ran;
+      for (auto& kv : sv->cls)
+        for (Chunk* c : kv.second) {
+          if (c->active == 0) continue;
+          const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c->ms_per_step)));
+          ensure_primed(c->s);
+          hipchk(hipEventRecord(c->e0, c->s->stream), "hipEventRecord");
+          for (int i = 0; i < k; ++i) step_once(c->s, false);
+          hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
+          ran.push_back({c, k});
+        }
+      // 3. poll: per-slot device flags tell which requests retired
       for (auto& pr : ran) {
-        Class& c = *pr.first;
+        Chunk& c = *pr.first;
         sync(c.s);
         float ms = 0.f;
         hipchk(hipEventElapsedTime(&ms, c.e0, c.e1), "hipEventElapsedTime");
@@ -364,11 +458,8 @@ static void server_free(Server* sv) {
   sv->stop = true;
   sv->cv.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
-  for (auto& kv : sv->cls) {
-    if (kv.second.e0) (void)hipEventDestroy(kv.second.e0);
-    if (kv.second.e1) (void)hipEventDestroy(kv.second.e1);
-    solver_free(kv.second.s);
-  }
+  for (auto& kv : sv->cls)
+    for (Chunk* c : kv.second) free_chunk(c);
   delete sv;
 }
 
@@ -395,38 +486,31 @@ int s2t_server_create(const int* sizes, const int* slots, int nclasses, double b
                       s2t_server** out) {
   if (!out) return api_bad("s2t_server_create: out is NULL");
   *out = nullptr;
-  if (!sizes || !slots || nclasses <= 0) return api_bad("s2t_server_create: no classes");
+  if (nclasses < 0 || (nclasses > 0 && (!sizes || !slots)))
+    return api_bad("s2t_server_create: bad class list");
+  if (!(budget_bytes >= 0)) return api_bad("s2t_server_create: budget must be >= 0");
   size_t need = 0;
   for (int i = 0; i < nclasses; ++i) {
     const std::string e = size_error(sizes[i]);
     if (!e.empty()) return api_bad(e);
     if (slots[i] <= 0) return api_bad("slot counts must be positive");
-    need += class_bytes(sizes[i], slots[i]);
+    for (int j = 0; j < i; ++j)
+      if (sizes[j] == sizes[i]) return api_bad("duplicate class size");
+    need += chunk_bytes(sizes[i], slots[i]);
   }
   if (budget_bytes > 0 && (double)need > budget_bytes)
-    return api_bad("slot capacity needs " + std::to_string(need) + " bytes, over the budget of " +
-                   std::to_string((size_t)budget_bytes));
+    return api_bad("the first chunk of each class needs " + std::to_string(need) +
+                   " bytes, over the budget of " + std::to_string((size_t)budget_bytes));
   Server* sv = new Server();
   S2T_API_BEGIN
   sv->budget = (size_t)budget_bytes;
   for (int i = 0; i < nclasses; ++i) {
-    if (sv->cls.count(sizes[i])) throw Fail(S2T_EINVAL, "duplicate class size");
-    Class c;
-    c.s = solver_new(sizes[i], 1, slots[i], false);
-    c.slot_req.assign(slots[i], -1);
-    c.slot_phase.assign(slots[i], 0);
-    c.bytes = class_bytes(sizes[i], slots[i]);
-    hipchk(hipEventCreate(&c.e0), "hipEventCreate");
-    hipchk(hipEventCreate(&c.e1), "hipEventCreate");
-    // a new class starts with every slot idle
-    for (int k = 0; k < slots[i]; ++k) {
-      int zero = 0;
-      hipchk(hipMemcpy(&c.s->inst[k].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
-             "hipMemcpy");
-    }
-    sv->used += c.bytes;
-    sv->cls[sizes[i]] = c;
+    sv->chunk_slots[sizes[i]] = slots[i];
+    Chunk* c = new_chunk(sizes[i], slots[i]);
+    sv->cls[sizes[i]].push_back(c);
+    sv->used += c->bytes;
   }
+  sv->peak = sv->used;
   sv->worker = std::thread(worker_loop, sv);
   *out = reinterpret_cast<s2t_server*>(sv);
   return S2T_OK;
@@ -465,7 +549,10 @@ int s2t_server_submit(s2t_server* h, const s2t_request* r, uint64_t* id) {
   rc.keep_state = r->keep_state != 0;
   sv->submitted += 1;
   if (sv->t_first == 0) sv->t_first = rc.t_submit;
-  const bool reject = !sv->cls.count(r->n);
+  // any supported size is admitted against the budget; a size none of whose instances fits in
+  // the budget at all is rejected
+  const bool reject = !size_error(r->n).empty() ||
+                      (sv->budget > 0 && chunk_bytes(r->n, 1) > sv->budget);
   if (reject) {
     rc.status = S2T_REQ_REJECTED;
     sv->rejected += 1;
@@ -593,6 +680,7 @@ int s2t_server_get_stats(s2t_server* h, s2t_server_stats* out) {
   out->mpts_steps_per_s = span > 0 ? sv->work_pts_steps / span / 1e6 : 0.0;
   out->budget_bytes = (double)sv->budget;
   out->used_bytes = (double)sv->used;
+  out->peak_bytes = (double)sv->peak;
   return S2T_OK;
 }
 

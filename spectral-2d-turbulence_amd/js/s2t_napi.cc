@@ -1294,6 +1294,9 @@ napi_value ServerStats(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, obj, "p99Latency", make_num(env, st.p99_latency_s));
   napi_set_named_property(env, obj, "meanLatency", make_num(env, st.mean_latency_s));
   napi_set_named_property(env, obj, "mptsStepsPerS", make_num(env, st.mpts_steps_per_s));
+  napi_set_named_property(env, obj, "budgetBytes", make_num(env, st.budget_bytes));
+  napi_set_named_property(env, obj, "usedBytes", make_num(env, st.used_bytes));
+  napi_set_named_property(env, obj, "peakBytes", make_num(env, st.peak_bytes));
   return obj;
 }
 

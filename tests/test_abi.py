@@ -97,7 +97,8 @@ def test_argument_checks_before_any_device_work(s2t):
     assert lib.s2t_server_create(sizes, slots, 2, 0.0, ctypes.byref(h)) == s2t.S2T_EINVAL
     assert "slot counts" in lib.s2t_last_error().decode()
     slots = (ctypes.c_int * 2)(4, 4)
-    assert lib.s2t_server_create(sizes, slots, 0, 0.0, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert lib.s2t_server_create(sizes, slots, -1, 0.0, ctypes.byref(h)) == s2t.S2T_EINVAL
+    assert lib.s2t_server_create(sizes, slots, 2, -1.0, ctypes.byref(h)) == s2t.S2T_EINVAL
     assert lib.s2t_server_create(sizes, slots, 2, 1e6, ctypes.byref(h)) == s2t.S2T_EINVAL
     assert "over the budget" in lib.s2t_last_error().decode()
     # null handles everywhere

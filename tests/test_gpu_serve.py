@@ -52,20 +52,62 @@ def test_server_matches_plain_solver(s2t):
         assert np.array_equal(got, s.read_state())
 
 
+def chunk_bytes(n, slots):
+    """Device bytes the server charges for a chunk of `slots` instances of size n: 3 states and
+    5 mixed planes per instance, 3 full fields and a state of scratch per chunk."""
+    k = n // 3
+    kc, nj = k + 1, 2 * k + 1
+    state = kc * ((nj + 63) // 64 * 64) * 8
+    plane = n * ((kc + 15) // 16 * 16) * 8
+    return slots * (3 * state + 5 * plane) + n * n * 24 + state
+
+
 def test_rejection_and_budget(s2t):
-    with s2t.Server({64: 2}) as sv:
-        rid = sv.submit(n=128, steps=5, **PHYS)
-        st, _, lat = sv.poll(rid)
-        assert st == s2t.REQ_REJECTED and lat < 0
-        sv.drain(10)
-        assert sv.stats()["rejected"] == 1
+    """Unsupported sizes, and sizes of which not one instance fits the budget, are rejected; any
+    other size is served whether or not a class was configured for it."""
+    with s2t.Server({64: 2}, budget_bytes=1e6) as sv:
+        big = sv.submit(n=2048, steps=5, **PHYS)
+        odd = sv.submit(n=96, steps=5, **PHYS)
+        other = sv.submit(n=128, steps=5, **PHYS)
+        for rid in (big, odd):
+            st, _, lat = sv.poll(rid)
+            assert st == s2t.REQ_REJECTED and lat < 0
+        sv.drain(60)
+        assert sv.poll(other)[0] == s2t.REQ_DONE
+        st = sv.stats()
+        assert st["rejected"] == 2 and st["completed"] == 1 and st["peak_bytes"] <= 1e6
     with pytest.raises(s2t.S2TError, match="over the budget"):
         s2t.Server({2048: 8}, budget_bytes=1e6)
 
 
+def test_admission_against_the_budget(s2t):
+    """Chunks are allocated as requests need them and freed when idle to make room for another
+    size: with a budget that holds one 4-slot chunk of 128^2, six 128^2 requests and three 64^2
+    requests (64 has no configured class) all complete, the server never holds more than the
+    budget, and results equal the plain solver's."""
+    budget = 1.05 * chunk_bytes(128, 4)
+    with s2t.Server({128: 4}, budget_bytes=budget) as sv:
+        ids = [sv.submit(n=128, steps=20 + k, seed=k, keep_state=k == 0, **PHYS) for k in range(6)]
+        ids += [sv.submit(n=64, steps=15 + k, seed=10 + k, keep_state=k == 0, **PHYS)
+                for k in range(3)]
+        sv.drain(120)
+        for rid in ids:
+            assert sv.poll(rid)[0] == s2t.REQ_DONE
+        got = {128: sv.read_state(ids[0]), 64: sv.read_state(ids[6])}
+        st = sv.stats()
+    assert st["completed"] == 9 and 0 < st["peak_bytes"] <= budget
+    for n, steps, seed in ((128, 20, 0), (64, 15, 10)):
+        with s2t.Solver(n) as s:
+            s.configure(**PHYS)
+            s.seed_spectrum(k0=14.0, amplitude=6.0, seed=seed)
+            s.step(steps)
+            assert np.array_equal(got[n], s.read_state())
+
+
 def test_queueing_beyond_slots(s2t):
-    """More requests than slots wait for slots to free and all complete."""
-    with s2t.Server({64: 2}) as sv:
+    """With room for one 2-slot chunk, more requests than slots wait for slots to free and all
+    complete."""
+    with s2t.Server({64: 2}, budget_bytes=1.05 * chunk_bytes(64, 2)) as sv:
         ids = [sv.submit(n=64, steps=10 + k, seed=k, **PHYS) for k in range(7)]
         sv.drain(60)
         for rid in ids:
@@ -87,11 +129,11 @@ def test_cancel_and_completion_callbacks(s2t):
             seen[rid] = status
 
     seed = dict(scenario="turbulence", k0=6.0, amplitude=5.0, seed=11)
-    with s2t.Server({64: 1}) as sv:
+    with s2t.Server({64: 1}, budget_bytes=1.05 * chunk_bytes(64, 1)) as sv:
         sv.set_callback(on_final)
         long_id = sv.submit(n=64, steps=10 ** 7, **seed, **PHYS)   # runs until cancelled
         queued = sv.submit(n=64, steps=10, **seed, **PHYS)        # waits for the one slot
-        rejected = sv.submit(n=128, steps=5, **PHYS)              # no class of that size
+        rejected = sv.submit(n=2048, steps=5, **PHYS)             # cannot fit in the budget
         t0 = time.time()
         while sv.poll(long_id)[0] != s2t.REQ_RUNNING:
             assert time.time() - t0 < 60
@@ -132,7 +174,7 @@ def test_worker_failure_fails_every_outstanding_request(s2t):
             if len(seen) == 3:
                 done.set()
 
-    sv = s2t.Server({64: 1})
+    sv = s2t.Server({64: 1}, budget_bytes=1.05 * chunk_bytes(64, 1))
     try:
         sv.inject_failure(1)
         sv.set_callback(on_final)

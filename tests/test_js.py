@@ -66,10 +66,11 @@ const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e
 const seed = { scenario: 'turbulence', k0: 6, amplitude: 5, seed: 11 };
 const settle = (p) => p.then((r) => ({ ok: r }), (e) => ({ err: e.message, status: e.status }));
 (async () => {
-  const sv = new m.Server([{ n: 64, slots: 1 }]);
+  // room for one one-slot chunk of 64^2 (225,280 bytes)
+  const sv = new m.Server([{ n: 64, slots: 1 }], { budgetBytes: 3e5 });
   const long = sv.submit({ n: 64, steps: 1e7, ...seed }, phys);       // runs until cancelled
   const queued = sv.submit({ n: 64, steps: 10, ...seed }, phys);      // waits for the one slot
-  const rejected = sv.submit({ n: 128, steps: 5, ...seed }, phys);    // no class of that size
+  const rejected = sv.submit({ n: 2048, steps: 5, ...seed }, phys);   // cannot fit in the budget
   const t0 = Date.now();
   while (sv.poll(long.id).status !== m.REQ.RUNNING) {
     if (Date.now() - t0 > 60000) throw new Error('never admitted');
@@ -227,7 +228,7 @@ FAIL_JS = r"""
 const m = require('./index.js');
 const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };
 (async () => {
-  const sv = new m.Server([{ n: 64, slots: 1 }]);
+  const sv = new m.Server([{ n: 64, slots: 1 }], { budgetBytes: 3e5 });
   sv.injectFailure(1);
   const reqs = [0, 1, 2].map((k) => sv.submit({ n: 64, steps: 20, seed: k }, phys));
   const out = await Promise.all(reqs.map((r) => r.result.then(

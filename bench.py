@@ -369,7 +369,7 @@ def bench_decaying(s2t, steps, warmup):
                          "frac": round(dye_gbs / HBM_PEAK_GBS, 4)},
             "kernels_us": {"k_rows": round(avg_us[s2t.PROF_ROWS], 2),
                            "k_cols": round(avg_us[s2t.PROF_COLS], 2),
-                           "dye (3 launches)": round(avg_us[s2t.PROF_DYE], 2)}}
+                           "dye (fused)": round(avg_us[s2t.PROF_DYE], 2)}}
 
 
 def dissipation_rate(spectrum, n, params):

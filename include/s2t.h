@@ -204,6 +204,10 @@ int s2t_write_velocity(s2t_solver* h, const float* uv);
  * takes the step of the last solver step from the device (dtbuf, dye.wgsl:63); a given dt > 0 is
  * used as is. Enqueues only. */
 int s2t_dye_step(s2t_solver* h, double dt);
+/* The same dye update as three launches, one per pass of dye.wgsl:58-113 (advect_fwd, advect_back,
+ * advect_corr), with the intermediate fields stored as textures: the fused update of
+ * s2t_dye_step, bit for bit (test hook). Enqueues only. */
+int s2t_dye_step_passes(s2t_solver* h, double dt);
 /* Blank all three dye textures and reset their parity (part of Solver.reset, solver.js:420-436,
  * before paintDye). Waits. */
 int s2t_reset_dye(s2t_solver* h);
@@ -389,7 +393,7 @@ int s2t_profile_sampled(s2t_solver* h, unsigned mask, int every);
 int s2t_profile_read(s2t_solver* h, int kind, int* launches, double* total_ms);
 /* Algorithmic HBM bytes of one launch of a kernel kind for this solver's size and the current
  * options (stage 0 of a step when the kinds differ by stage); for S2T_PROF_DYE, of one dye step
- * (its three launches, 56 B per dye texel + 24 B per grid point). */
+ * (the fused update: 16 B per dye texel + 8 B per grid point). */
 int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes);
 /* Measurement helper with no counterpart in the reference: `reps` launches of a plain 16-byte
  * streaming copy kernel (k_copy) of `bytes` bytes on a private stream; average ms per copy. Its

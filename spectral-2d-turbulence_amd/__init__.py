@@ -49,6 +49,7 @@ EXPORTS = [
     "s2t_primed", "s2t_encode_diagnostics", "s2t_copy_diagnostics",
     "s2t_read_copied_diagnostics", "s2t_buffer_create", "s2t_buffer_destroy", "s2t_buffer_write",
     "s2t_buffer_read", "s2t_buffer_copy", "s2t_fft2_device", "s2t_server_inject_failure",
+    "s2t_dye_step_passes",
 ]
 SCENARIO = {"turbulence": 0, "taylor-green": 1, "dipole": 2, "merger": 3}
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4, 5
@@ -190,6 +191,7 @@ def load(path=None):
         "s2t_buffer_copy": (ip, [vp, ctypes.c_size_t, vp, ctypes.c_size_t, ctypes.c_size_t]),
         "s2t_fft2_device": (ip, [vp, vp, ip]),
         "s2t_server_inject_failure": (ip, [vp, ip]),
+        "s2t_dye_step_passes": (ip, [vp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -351,6 +353,10 @@ class Solver:
     def dye_step(self, dt=0.0):
         """One dye update (dt <= 0: the last step's dt on the device); enqueues only."""
         _check(self.lib, self.lib.s2t_dye_step(self.h, float(dt)))
+
+    def dye_step_passes(self, dt=0.0):
+        """The same update as three launches (the unfused passes; test hook); enqueues only."""
+        _check(self.lib, self.lib.s2t_dye_step_passes(self.h, float(dt)))
 
     def reset_dye(self):
         _check(self.lib, self.lib.s2t_reset_dye(self.h))

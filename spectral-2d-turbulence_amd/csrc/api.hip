@@ -290,6 +290,17 @@ int s2t_dye_step(s2t_solver* h, double dt) {
   S2T_API_END
 }
 
+int s2t_dye_step_passes(s2t_solver* h, double dt) {
+  SOLVER(h);
+  if (!s->vel) return api_bad("s2t_dye_step_passes: no velocity (s2t_write_velocity or a dye step)");
+  if (!std::isfinite(dt)) return api_bad("s2t_dye_step_passes: dt must be finite");
+  S2T_API_BEGIN
+  dye_step_passes(s->stream, s->dye, s->dye_parity, s->vel, dt > 0 ? nullptr : &s->inst[0].st,
+                  dt, s->phys_slot[0].dye_fade, s->splats, s->n, s->m);
+  s->dye_parity = 1 - s->dye_parity;
+  S2T_API_END
+}
+
 int s2t_reset_dye(s2t_solver* h) {
   SOLVER(h);
   S2T_API_BEGIN
@@ -586,11 +597,10 @@ int s2t_kernel_bytes(s2t_solver* h, int kind, int stage, double* bytes) {
     b = 2 * plane + st * (stage == 0 ? 2 : 3) + 3 * plane;
     if (force) b += stage == 0 ? plane + st : st;
   } else if (kind == S2T_PROF_DYE) {
-    // one BFECC dye step, its three launches together: forward and backward advection each read
-    // and write one rgba16f texture (8 B per texel); the correction reads phi0 and phi_hat and
-    // writes the result; each launch reads the N x N float2 velocity once
+    // one BFECC dye step (the fused kernel): phi0 read and the result written once (rgba16f,
+    // 8 B per texel each), the N x N float2 velocity read once; phi1 and phi_hat stay on chip
     const double m = s->m;
-    b = 7.0 * m * m * 8.0 + 3.0 * n * n * 8.0;
+    b = 2.0 * m * m * 8.0 + n * n * 8.0;
   } else {
     return api_bad("no byte model for this kernel kind");
   }

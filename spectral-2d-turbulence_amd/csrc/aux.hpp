@@ -25,6 +25,9 @@ void diagnostics(hipStream_t st, const float2* state, const double2* wtab, doubl
                  double* zonal, int n);
 void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
               const DevState* dev, double dt, float fade, const Splats& sp, int n, int m);
+// the same update as three launches (the unfused passes; test hook)
+void dye_step_passes(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
+                     const DevState* dev, double dt, float fade, const Splats& sp, int n, int m);
 void dye_paint(hipStream_t st, ushort4* dst, int m);
 void copy_probe(hipStream_t st, const void* src, void* dst, size_t bytes);
 

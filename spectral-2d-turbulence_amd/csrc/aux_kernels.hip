@@ -436,40 +436,52 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
   return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
                      a.w + (b.w - a.w) * f);
 }
-__device__ __forceinline__ float4 sample_tex(const ushort4* __restrict__ tex, int m, float px,
-                                             float py) {
+// Bilinear sample at unit position (px, py) of a texel source `at(ix, iy)` (float4; repeat
+// addressing is the source's), texel centres at (i + 0.5)/m.
+template <class At>
+__device__ __forceinline__ float4 sample_at(At&& at, int m, float px, float py) {
   const float gx = px * (float)m - 0.5f, gy = py * (float)m - 0.5f;
   const float bx = floorf(gx), by = floorf(gy);
   const float fx = gx - bx, fy = gy - by;
   const int ix = (int)bx, iy = (int)by;
-  const float4 top = lerp4(texel(tex, m, ix, iy), texel(tex, m, ix + 1, iy), fx);
-  const float4 bot = lerp4(texel(tex, m, ix, iy + 1), texel(tex, m, ix + 1, iy + 1), fx);
+  const float4 top = lerp4(at(ix, iy), at(ix + 1, iy), fx);
+  const float4 bot = lerp4(at(ix, iy + 1), at(ix + 1, iy + 1), fx);
   return lerp4(top, bot, fy);
+}
+__device__ __forceinline__ float4 sample_tex(const ushort4* __restrict__ tex, int m, float px,
+                                             float py) {
+  return sample_at([&](int ix, int iy) { return texel(tex, m, ix, iy); }, m, px, py);
 }
 __device__ __forceinline__ float2 vel_at(const float2* __restrict__ vel, int n, int ix, int iy) {
   const int x = ix & (n - 1), y = iy & (n - 1);
   return vel[(size_t)y * n + x];
 }
-__device__ __forceinline__ float2 sample_vel(const float2* __restrict__ vel, int n, float px,
-                                             float py) {
+// bilinear velocity at unit position (px, py) from a cell source `va(ix, iy)` (cell centres at
+// (i + 0.5)/n, repeat addressing is the source's)
+template <class Va>
+__device__ __forceinline__ float2 sample_vel_at(Va&& va, int n, float px, float py) {
   const float gx = px * (float)n - 0.5f, gy = py * (float)n - 0.5f;
   const float bx = floorf(gx), by = floorf(gy);
   const float fx = gx - bx, fy = gy - by;
   const int ix = (int)bx, iy = (int)by;
-  const float2 v00 = vel_at(vel, n, ix, iy), v10 = vel_at(vel, n, ix + 1, iy);
-  const float2 v01 = vel_at(vel, n, ix, iy + 1), v11 = vel_at(vel, n, ix + 1, iy + 1);
+  const float2 v00 = va(ix, iy), v10 = va(ix + 1, iy);
+  const float2 v01 = va(ix, iy + 1), v11 = va(ix + 1, iy + 1);
   const float2 a = make_float2(v00.x + (v10.x - v00.x) * fx, v00.y + (v10.y - v00.y) * fx);
   const float2 b = make_float2(v01.x + (v11.x - v01.x) * fx, v01.y + (v11.y - v01.y) * fx);
   return make_float2(a.x + (b.x - a.x) * fy, a.y + (b.y - a.y) * fy);
 }
 __device__ __forceinline__ float fract1(float x) { return x - floorf(x); }
 // RK2 midpoint back-trace (dye.wgsl:40-45)
+template <class Va>
+__device__ __forceinline__ float2 trace_at(Va&& va, int n, float px, float py, float h) {
+  const float2 v1 = sample_vel_at(va, n, px, py);
+  const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
+  const float2 v2 = sample_vel_at(va, n, mx, my);
+  return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+}
 __device__ __forceinline__ float2 trace(const float2* __restrict__ vel, int n, float px, float py,
                                         float h) {
-  const float2 v1 = sample_vel(vel, n, px, py);
-  const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
-  const float2 v2 = sample_vel(vel, n, mx, my);
-  return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+  return trace_at([=](int ix, int iy) { return vel_at(vel, n, ix, iy); }, n, px, py, h);
 }
 __device__ __forceinline__ ushort4 to_half4(float4 v) {
   return make_ushort4(__half_as_ushort(__float2half(v.x)), __half_as_ushort(__float2half(v.y)),
@@ -504,24 +516,11 @@ __global__ void k_dye_advect(const ushort4* __restrict__ src, ushort4* __restric
   dst[(size_t)y * m + x] = to_half4(sample_tex(src, m, s.x, s.y));
 }
 
-// advect_corr, dye.wgsl:76-113
-__global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __restrict__ aux,
-                           ushort4* __restrict__ dst, const float2* __restrict__ vel,
-                           const DevState* dev, double dt, float fade, Splats sp, int n, int m) {
-  const int2 xy = dye_texel();
-  const int x = xy.x, y = xy.y;
-  if (x >= m || y >= m) return;
-  const float h = (float)dye_dt(dev, dt) / TWO_PI_F;
-  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
-  const float2 s = trace(vel, n, px, py, h);
-  const float4 a = sample_tex(src, m, s.x, s.y);
-  const float4 b = sample_tex(aux, m, s.x, s.y);
-  float4 corr = make_float4(1.5f * a.x - 0.5f * b.x, 1.5f * a.y - 0.5f * b.y,
-                            1.5f * a.z - 0.5f * b.z, 1.5f * a.w - 0.5f * b.w);
-  const float gx = s.x * (float)m - 0.5f, gy = s.y * (float)m - 0.5f;
-  const int ix = (int)floorf(gx), iy = (int)floorf(gy);
-  const float4 c00 = texel(src, m, ix, iy), c10 = texel(src, m, ix + 1, iy);
-  const float4 c01 = texel(src, m, ix, iy + 1), c11 = texel(src, m, ix + 1, iy + 1);
+// The MacCormack limiter (clamp to the four phi0 texels around the foot), fade, splat dye and
+// alpha 1 (dye.wgsl:86-112)
+__device__ __forceinline__ float4 dye_correct(float4 corr, float4 c00, float4 c10, float4 c01,
+                                              float4 c11, float fade, const Splats& sp, float px,
+                                              float py) {
   const float4 lo = make_float4(fminf(fminf(c00.x, c10.x), fminf(c01.x, c11.x)),
                                 fminf(fminf(c00.y, c10.y), fminf(c01.y, c11.y)),
                                 fminf(fminf(c00.z, c10.z), fminf(c01.z, c11.z)),
@@ -547,11 +546,244 @@ __global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __res
     }
   }
   res.w = 1.f;
-  dst[(size_t)y * m + x] = to_half4(res);
+  return res;
+}
+
+// advect_corr, dye.wgsl:76-113
+__global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __restrict__ aux,
+                           ushort4* __restrict__ dst, const float2* __restrict__ vel,
+                           const DevState* dev, double dt, float fade, Splats sp, int n, int m) {
+  const int2 xy = dye_texel();
+  const int x = xy.x, y = xy.y;
+  if (x >= m || y >= m) return;
+  const float h = (float)dye_dt(dev, dt) / TWO_PI_F;
+  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+  const float2 s = trace(vel, n, px, py, h);
+  const float4 a = sample_tex(src, m, s.x, s.y);
+  const float4 b = sample_tex(aux, m, s.x, s.y);
+  float4 corr = make_float4(1.5f * a.x - 0.5f * b.x, 1.5f * a.y - 0.5f * b.y,
+                            1.5f * a.z - 0.5f * b.z, 1.5f * a.w - 0.5f * b.w);
+  const float gx = s.x * (float)m - 0.5f, gy = s.y * (float)m - 0.5f;
+  const int ix = (int)floorf(gx), iy = (int)floorf(gy);
+  const float4 c00 = texel(src, m, ix, iy), c10 = texel(src, m, ix + 1, iy);
+  const float4 c01 = texel(src, m, ix, iy + 1), c11 = texel(src, m, ix + 1, iy + 1);
+  dst[(size_t)y * m + x] = to_half4(dye_correct(corr, c00, c10, c01, c11, fade, sp, px, py));
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// The BFECC update fused into one kernel over 32 x 32 tiles of the dye. A tile's output needs
+// phi_hat at the trace feet of its texels, phi_hat needs phi1 at its own back-trace feet and phi1
+// needs phi0 at its forward-trace feet. A foot less than one texel from its texel centre lies in
+// the bilinear stencil of that texel and its neighbours, so phi0, phi1 and phi_hat are staged in
+// LDS on the tile grown by 3, 2 and 1 texels (phi0 loaded, phi1 and phi_hat computed there,
+// rounded to binary16 like the three-pass textures), with the velocity cells under the traced
+// points. CFL 0.5 on twice the spectral grid moves a point at most one texel per step, and less
+// wherever |u| and |v| do not peak together. Each tile checks the bound on its staged velocity;
+// where it fails (that maximum, a dt_min clamp, a fixed dt) every sample outside a staged region
+// is computed on the spot from the same functions, so the result is the three-pass result bit
+// for bit whatever the step.
+// ---------------------------------------------------------------------------------------------
+constexpr int DYE_T = 32;
+constexpr int DYE_E2 = 1, DYE_E1 = 2, DYE_E0 = 3;
+constexpr int DYE_R0 = DYE_T + 2 * DYE_E0, DYE_R1 = DYE_T + 2 * DYE_E1, DYE_R2 = DYE_T + 2 * DYE_E2;
+constexpr int DYE_THREADS = 256;
+// velocity cells staged for dye scale r: the cells under the points traced (the phi1 region, the
+// widest) and their midpoints, with a cell of margin each side
+template <int R>
+struct DyeVel {
+  static constexpr int EV = DYE_E1 + 1;                  // dye texels around the tile
+  static constexpr int LO = (EV + R - 1) / R + 1;        // cells before the tile's first cell
+  static constexpr int CELLS = (DYE_T + R - 1) / R + 2 * LO + 1;
+};
+
+__device__ __forceinline__ float4 half4(ushort4 v) {
+  return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
+                     __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
+}
+
+// The stages of one tile, after phi0 and the velocity cells are staged. CHECKED: every lookup
+// checks its staged region and computes a sample outside it on the spot; otherwise (the tile's
+// feet are known to stay within one texel) lookups go straight to LDS.
+template <int R, bool CHECKED>
+__device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
+                                         const float2* __restrict__ vel, const ushort4* s0,
+                                         ushort4* s1, ushort4* s2, const float2* sv,
+                                         float2* sfoot, float hf,
+                                         float hb, float fade, const Splats& sp, int n, int m,
+                                         int x0, int y0, int vx0, int vy0) {
+  using V = DyeVel<R>;
+  const int tid = (int)threadIdx.x;
+  const int mask = m - 1, nmask = n - 1;
+  auto va = [=](int ix, int iy) -> float2 {
+    const int rx = (ix - vx0) & nmask, ry = (iy - vy0) & nmask;
+    if (!CHECKED || (rx < V::CELLS && ry < V::CELLS)) return sv[ry * V::CELLS + rx];
+    return vel_at(vel, n, ix, iy);
+  };
+  auto phi0 = [=](int ix, int iy) -> float4 {
+    const int rx = (ix - x0 + DYE_E0) & mask, ry = (iy - y0 + DYE_E0) & mask;
+    if (!CHECKED || (rx < DYE_R0 && ry < DYE_R0)) return half4(s0[ry * DYE_R0 + rx]);
+    return texel(src, m, ix, iy);
+  };
+  auto centre = [=](int i) { return ((float)(i & mask) + 0.5f) / (float)m; };
+  // phi1 of texel (ix, iy): the forward advection of phi0, rounded to binary16
+  auto phi1_of = [=](int ix, int iy) -> ushort4 {
+    const float2 f = trace_at(va, n, centre(ix), centre(iy), hf);
+    return to_half4(sample_at(phi0, m, f.x, f.y));
+  };
+  // the same at a point of the phi1 region, keeping the forward foot of the tile's own texels
+  // (the trace of the correction stage)
+  auto phi1_keep = [=](int i) -> ushort4 {
+    const int rx = i % DYE_R1, ry = i / DYE_R1;
+    const int ix = x0 - DYE_E1 + rx, iy = y0 - DYE_E1 + ry;
+    const float2 f = trace_at(va, n, centre(ix), centre(iy), hf);
+    const int tx = rx - DYE_E1, ty = ry - DYE_E1;
+    if (tx >= 0 && tx < DYE_T && ty >= 0 && ty < DYE_T) sfoot[ty * DYE_T + tx] = f;
+    return to_half4(sample_at(phi0, m, f.x, f.y));
+  };
+  auto phi1 = [=](int ix, int iy) -> float4 {
+    const int rx = (ix - x0 + DYE_E1) & mask, ry = (iy - y0 + DYE_E1) & mask;
+    if (!CHECKED || (rx < DYE_R1 && ry < DYE_R1)) return half4(s1[ry * DYE_R1 + rx]);
+    return half4(phi1_of(ix, iy));
+  };
+  // phi1 without the staged values (once phi_hat has taken their space)
+  auto phi1_fresh = [=](int ix, int iy) -> float4 { return half4(phi1_of(ix, iy)); };
+  // phi_hat of texel (ix, iy): the backward advection of phi1 (from the accessor p1), rounded to
+  // binary16
+  auto phih_from = [=](auto p1, int ix, int iy) -> ushort4 {
+    const float2 f = trace_at(va, n, centre(ix), centre(iy), hb);
+    return to_half4(sample_at(p1, m, f.x, f.y));
+  };
+  auto phih_of = [=](int ix, int iy) -> ushort4 { return phih_from(phi1, ix, iy); };
+  auto phih = [=](int ix, int iy) -> float4 {
+    const int rx = (ix - x0 + DYE_E2) & mask, ry = (iy - y0 + DYE_E2) & mask;
+    if (!CHECKED || (rx < DYE_R2 && ry < DYE_R2)) return half4(s2[ry * DYE_R2 + rx]);
+    return half4(phih_from(phi1_fresh, ix, iy));
+  };
+  // two items per thread per iteration: independent chains the scheduler can interleave
+  for (int i = tid; i < DYE_R1 * DYE_R1; i += 2 * DYE_THREADS) {
+    const int k = i + DYE_THREADS;
+    const ushort4 a = phi1_keep(i);
+    if (k < DYE_R1 * DYE_R1) s1[k] = phi1_keep(k);
+    s1[i] = a;
+  }
+  __syncthreads();
+  // phi_hat into registers (all of a thread's points at once: independent chains), then over
+  // phi1's LDS space once every thread is done reading phi1
+  constexpr int NH = (DYE_R2 * DYE_R2 + DYE_THREADS - 1) / DYE_THREADS;
+  ushort4 hv[NH];
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    const int i = tid + k * DYE_THREADS;
+    if (i < DYE_R2 * DYE_R2) hv[k] = phih_of(x0 - DYE_E2 + i % DYE_R2, y0 - DYE_E2 + i / DYE_R2);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    const int i = tid + k * DYE_THREADS;
+    if (i < DYE_R2 * DYE_R2) s2[i] = hv[k];
+  }
+  __syncthreads();
+  auto out = [&](int i) {
+    const int x = x0 + i % DYE_T, y = y0 + i / DYE_T;
+    if (x >= m || y >= m) return;
+    const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+    const float2 s = sfoot[i];  // trace_at(va, n, px, py, hf), from the phi1 stage
+    const float4 a = sample_at(phi0, m, s.x, s.y);
+    const float4 bb = sample_at(phih, m, s.x, s.y);
+    const float4 corr = make_float4(1.5f * a.x - 0.5f * bb.x, 1.5f * a.y - 0.5f * bb.y,
+                                    1.5f * a.z - 0.5f * bb.z, 1.5f * a.w - 0.5f * bb.w);
+    const float gx = s.x * (float)m - 0.5f, gy = s.y * (float)m - 0.5f;
+    const int ix = (int)floorf(gx), iy = (int)floorf(gy);
+    const float4 c00 = phi0(ix, iy), c10 = phi0(ix + 1, iy);
+    const float4 c01 = phi0(ix, iy + 1), c11 = phi0(ix + 1, iy + 1);
+    dst[(size_t)y * m + x] = to_half4(dye_correct(corr, c00, c10, c01, c11, fade, sp, px, py));
+  };
+  for (int i = tid; i < DYE_T * DYE_T; i += 2 * DYE_THREADS) {
+    out(i);
+    if (i + DYE_THREADS < DYE_T * DYE_T) out(i + DYE_THREADS);
+  }
+}
+
+// R = m / n, the dye scale (1 or 2)
+template <int R>
+__global__ void __launch_bounds__(DYE_THREADS)
+k_dye_fused(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
+            const float2* __restrict__ vel, const DevState* dev, double dt, float fade, Splats sp,
+            int n, int m) {
+  using V = DyeVel<R>;
+  static_assert(DYE_THREADS % 64 == 0, "whole waves");
+  __shared__ ushort4 s0[DYE_R0 * DYE_R0];
+  // phi_hat (R2 x R2) takes phi1's space (R1 x R1) once phi1 is no longer read
+  __shared__ ushort4 s1[DYE_R1 * DYE_R1];
+  ushort4* s2 = s1;
+  static_assert(DYE_R2 <= DYE_R1, "phi_hat fits in phi1's space");
+  __shared__ float2 sv[V::CELLS * V::CELLS];
+  __shared__ float2 sfoot[DYE_T * DYE_T];
+  __shared__ float wmax[DYE_THREADS / 64];
+  // tiles: bands of tile rows per XCD (consecutive workgroups go round-robin over the 8 XCDs;
+  // neighbouring tiles share their halos through one L2)
+  const int tiles_x = (m + DYE_T - 1) / DYE_T, total = tiles_x * tiles_x;
+  int b = (int)blockIdx.x;
+  if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+  const int x0 = (b % tiles_x) * DYE_T, y0 = (b / tiles_x) * DYE_T;
+  const int tid = (int)threadIdx.x;
+  const int mask = m - 1, nmask = n - 1;
+  const int vx0 = x0 / R - V::LO, vy0 = y0 / R - V::LO;
+  const float dtf = (float)dye_dt(dev, dt);
+  const float hf = 1.f * dtf / TWO_PI_F, hb = -1.f * dtf / TWO_PI_F;
+
+  // stage the velocity cells (and their largest component) and phi0
+  float vmax = 0.f;
+  for (int i = tid; i < V::CELLS * V::CELLS; i += DYE_THREADS) {
+    const int rx = i % V::CELLS, ry = i / V::CELLS;
+    const float2 v = vel[(size_t)((vy0 + ry) & nmask) * n + ((vx0 + rx) & nmask)];
+    sv[i] = v;
+    vmax = fmaxf(vmax, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int i = tid; i < DYE_R0 * DYE_R0; i += DYE_THREADS) {
+    const int rx = i % DYE_R0, ry = i / DYE_R0;
+    s0[i] = src[(size_t)((y0 - DYE_E0 + ry) & mask) * m + ((x0 - DYE_E0 + rx) & mask)];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+  if ((tid & 63) == 0) wmax[tid >> 6] = vmax;
+  __syncthreads();
+  vmax = wmax[0];
+#pragma unroll
+  for (int w = 1; w < DYE_THREADS / 64; ++w) vmax = fmaxf(vmax, wmax[w]);
+  // Bilinear velocities never exceed the largest staged component, so every foot of the tile's
+  // traces (and every midpoint) lies within vmax |h| m texels of its point; below one texel, with
+  // room for rounding, all lookups stay in the staged regions. A non-finite velocity fails the
+  // test and takes the checked path.
+  const bool inside = vmax * fabsf(hf) * (float)m < 0.999f;
+  if (inside)
+    dye_tile<R, false>(src, dst, vel, s0, s1, s2, sv, sfoot, hf, hb, fade, sp, n, m, x0, y0, vx0,
+                       vy0);
+  else
+    dye_tile<R, true>(src, dst, vel, s0, s1, s2, sv, sfoot, hf, hb, fade, sp, n, m, x0, y0, vx0,
+                      vy0);
 }
 
 void dye_step(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
               const DevState* dev, double dt, float fade, const Splats& sp, int n, int m) {
+  if (n <= 0 || m <= 0 || (n & (n - 1)) != 0 || (m & (m - 1)) != 0 || (m != n && m != 2 * n))
+    throw Fail(S2T_EINVAL, "dye step: grid and dye sizes must be powers of two, dye = 1 or 2 x grid");
+  const int tiles_x = (m + DYE_T - 1) / DYE_T;
+  // bind-group parity of src/gpu/solver.js:299-303: read dye[p], write dye[1 - p]
+  if (m == 2 * n)
+    hipLaunchKernelGGL(k_dye_fused<2>, dim3(tiles_x * tiles_x), dim3(DYE_THREADS), 0, st,
+                       dye[parity], dye[1 - parity], vel, dev, dt, fade, sp, n, m);
+  else
+    hipLaunchKernelGGL(k_dye_fused<1>, dim3(tiles_x * tiles_x), dim3(DYE_THREADS), 0, st,
+                       dye[parity], dye[1 - parity], vel, dev, dt, fade, sp, n, m);
+  check_launch("k_dye_fused");
+}
+
+// The unfused passes, one launch each (the structure of dye.wgsl:58-113): the fused kernel's
+// result, bit for bit (test hook).
+void dye_step_passes(hipStream_t st, ushort4* const dye[3], int parity, const float2* vel,
+                     const DevState* dev, double dt, float fade, const Splats& sp, int n, int m) {
   if (n <= 0 || m <= 0 || (n & (n - 1)) != 0 || (m & (m - 1)) != 0)
     throw Fail(S2T_EINVAL, "dye step: grid and dye sizes must be powers of two");
   const dim3 blk(16, 16), grd((m + 15) / 16, (m + 15) / 16);

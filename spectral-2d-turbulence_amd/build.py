@@ -24,6 +24,11 @@ SOURCES = ["step_kernels.hip", "aux_kernels.hip", "core.hip", "api.hip", "batch.
 ARCH = os.environ.get("S2T_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# The step kernels fuse a multiply and an add into an FMA only within one expression: with fusion
+# across statements the compiler's choice depends on how often a product is used in the whole
+# kernel, so kernels that share the same per-value code (the column-group widths, the
+# field-parallel column kernel) could round differently; they give the same state bit for bit.
+FILE_FLAGS = {"step_kernels.hip": ["-ffp-contract=on"]}
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     # packed FP32 instructions are disabled for all device code
@@ -46,7 +51,8 @@ def build_lib(jobs=4, verbose=False):
 
     def one(src):
         obj = os.path.join(OBJ, src + ".o")
-        out = _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        out = _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+                    "-o", obj])
         if verbose and out.strip():
             print(out)
         return obj

@@ -285,7 +285,7 @@ int s2t_batch_column_width(s2t_batch* h, int member, int* width) {
   S2T_API_BEGIN
   int w[MAX_COL_WIDTHS];
   s->ops->col_widths(w);
-  *width = s->cols_c > 0 ? s->cols_c : w[0];
+  *width = s->cols_c != 0 ? s->cols_c : w[0];
   S2T_API_END
 }
 

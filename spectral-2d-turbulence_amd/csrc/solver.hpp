@@ -33,7 +33,7 @@ struct Ops {
   // count, at most MAX_COL_WIDTHS
   int (*col_widths)(int* out);
 };
-constexpr int MAX_COL_WIDTHS = 4;
+constexpr int MAX_COL_WIDTHS = 6;
 
 const Ops* ops_for(int n);
 
@@ -57,7 +57,7 @@ struct Solver {
   double2* wtab = nullptr;     // exp(2 pi i m / N), FP64, for the zonal mean
   Inst* inst = nullptr;        // per-instance parameters and scalars (batch entries)
   int batch = 1;               // instances sharing this solver's buffers
-  int cols_c = 0;              // columns per column-kernel workgroup in a step (0: the default)
+  int cols_c = 0;              // column-kernel width in a step (0: the default; < 0: field-parallel)
   double* diag_bins = nullptr; // [2][NBINS] energy and enstrophy per bin, then 2 totals
   double* diag_zonal = nullptr;
   double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation

@@ -113,6 +113,13 @@ struct ColWidths {
   static constexpr int n = 1;
   static constexpr int w[4] = {ColCfg<N>::C, 0, 0, 0};
 };
+// sizes with the field-parallel column kernel (k_cols_fp), which is then their default: one
+// instance steps in 54 / 100 us at 512^2 / 1024^2 against 64 / 122 us with the best column-group
+// width; at 2048^2 (206 against 167 us) the column groups already keep the SIMDs busy
+template <int N>
+struct ColFp {
+  static constexpr bool on = N == 512 || N == 1024;
+};
 template <> struct ColWidths<2048> { static constexpr int n = 3; static constexpr int w[4] = {1, 2, 4, 0}; };
 template <> struct ColWidths<1024> { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
 template <> struct ColWidths<512>  { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
@@ -717,6 +724,203 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// column kernel, field-parallel form (grids up to 2048^2, where one thread group per column
+// leaves most SIMDs idle: 342 one-wave workgroups at 1024^2)
+// ---------------------------------------------------------------------------------------------
+// One column per workgroup and three thread groups of T threads side by side: groups 0 and 1
+// transform P' and Q' (group 2 the pointer forcing at stage 0, else an idle transform that keeps
+// the barriers in step); P_hat and Q_hat both stay in LDS (no round trip through the output state);
+// the per-mode loop is split over the three groups by pairs of row blocks; then groups 0, 1, 2
+// transform omega', u', v' (s', d' after stage 2) side by side. Every value goes through the same
+// device code as in k_cols, so the state is the same bit for bit.
+constexpr int FP_GROUPS = 3;
+constexpr int FP_WIDTH = -FP_GROUPS;  // its code among the column-group widths
+
+template <int N, bool FORCE>
+__global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC, NJ = Ge::NJ;
+  constexpr int LSTRIDE = ColCfg<N>::LSTRIDE;
+  constexpr int G8 = (KC + 7) / 8 * 8;
+  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), STATE_B = (uint32_t)(Ge::STATE * 8);
+  constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;
+  static_assert(T % 64 == 0, "thread groups of whole waves");
+  __shared__ float2 lds_all[FP_GROUPS * LSTRIDE];
+
+  const int bid = blockIdx.x;
+  const int kx = (bid % 8) * (G8 / 8) + bid / 8;  // XCD-aware: a contiguous range of columns per XCD
+  if (kx >= KC) return;
+  Inst* I = a.inst + blockIdx.y;
+  if (!I->st.active) return;  // uniform over the block
+  const Phys ph = I->ph;
+  const size_t inst = blockIdx.y;
+  const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
+  const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
+  const Buf bw0 = make_buf(a.w0 + inst * a.s_state, STATE_B);
+  const Buf bwc = make_buf(a.wcur + inst * a.s_state, STATE_B);
+  const Buf bwo = make_buf(a.wout + inst * a.s_state, STATE_B);
+  const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
+  const Buf bfk = make_buf(FORCE ? a.fk + inst * a.s_state : a.fk, FORCE ? STATE_B : 0u);
+  const int tid = threadIdx.x;
+  const int g = tid / T, t = tid - g * T;
+  float2* ldsP = lds_all;
+  float2* ldsQ = lds_all + LSTRIDE;
+  float2* lds = lds_all + g * LSTRIDE;  // this group's exchange buffer
+  const float fkx = (float)kx;
+  const uint32_t vm = Ge::off(t, kx);
+  const uint32_t vs = ((uint32_t)kx * Ge::PITCH + (uint32_t)t) * 8u;
+  auto soff_s = [](int q) -> uint32_t {
+    return (uint32_t)(q * T - (2 * q >= PTS ? N - NJ : 0)) * 8u;
+  };
+  auto inband = [&](int q) -> bool {
+    const int i = t + q * T;
+    return 2 * q < PTS ? i <= K : i >= N - K;
+  };
+  auto qout = [](int q) constexpr { return ColBand<N>::out(q); };
+
+  double dt;
+  const uint32_t step = I->st.step;
+  if (a.stage == 0) {
+    __shared__ float speed_s;
+    if (tid < 64) {
+      float v = __uint_as_float(I->st.speed_bits[tid % DevState::SPEED_SLOTS]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+      if (tid == 0) speed_s = v;
+    }
+    __syncthreads();
+    const float speed = speed_s;
+    dt = ph.fixed_dt > 0.0 ? ph.fixed_dt : cfl_dt(speed, ph);
+    if (bid == 0 && tid == 0) {
+      I->st.dt = dt;
+      I->st.speed = speed;
+    }
+  } else {
+    dt = I->st.dt;
+  }
+
+  // forward column transforms side by side: P' (group 0), Q' (group 1), F' (group 2, stage 0 with
+  // pointer forcing; otherwise group 2 transforms zeros, so that every group meets the same
+  // barriers)
+  {
+    const bool force_now = FORCE && a.stage == 0;
+    const Buf src = g == 2 ? bf : bz;
+    const uint32_t base = g == 1 ? PLANE_B : 0u;
+    const uint32_t v = (g < 2 || force_now) ? vm : BUF_OOB;
+    float2 X[PTS];
+#pragma unroll
+    for (int i = 0; i < PTS; ++i) {
+      const int q = dft_load_order<PTS>(i);
+      X[q] = ld2(src, v, base + q * ROWS_B);
+    }
+    line_fft<N, -1>(X, lds, t, a.tw);
+    __syncthreads();
+    if (g < 2) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q)
+        if (!qout(q)) lds[swz(t + q * T)] = X[q];
+    } else if (force_now) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q)
+        if (inband(q)) st2(bfk, X[q], vs, soff_s(q));
+    }
+  }
+  __syncthreads();
+
+  // per-mode right-hand side and RK stage (as in k_cols), pairs of row blocks dealt to the groups
+  static_assert(PTS % 2 == 0, "the mode loop takes row blocks in pairs");
+  constexpr int SK0 = ColBand<N>::skip_first(), SKN = ColBand<N>::skip_count();
+  constexpr int NPI = (PTS - SKN) / 2;
+  auto qof = [](int pi) { return 2 * pi + (2 * pi >= SK0 ? SKN : 0); };
+  {
+    const float fdt = (float)dt;
+    const bool need_wc = a.stage != 0;
+    const float2 zero = make_float2(0.f, 0.f);
+    auto block = [&](int q) {
+      const bool inb = inband(q);
+      const uint32_t so = soff_s(q), v = inb ? vs : BUF_OOB;
+      const float2 w0v = ld2(bw0, v, so);
+      const float2 wcv = ld2(bwc, need_wc ? v : BUF_OOB, so);
+      const float2 fkv = FORCE ? ld2(bfk, v, so) : zero;
+      const int idx = t + q * T;
+      const float2 pv = ldsP[swz(idx)];
+      float2 wv = zero;
+      if (inb) {
+        const float fky = (float)wn(idx, N);
+        const float k2 = fkx * fkx + fky * fky;
+        if (k2 > 0.f) {
+          const float2 nh = a.sd ? nhat(pv, ldsQ[swz(idx)], 0.5f * (fkx + fky), 0.5f * (fkx - fky))
+                                 : nhat(pv, ldsQ[swz(idx)], fkx, fky);
+          float2 rhs = make_float2(-nh.x, -nh.y);
+          if constexpr (FORCE) rhs = cadd(rhs, fkv);
+          if (ph.force_amp > 0.f)
+            rhs = cadd(rhs, stoch_force(idx, kx, fkx, fky, k2, ph, step, fdt, N));
+          const IFactor e = ifactor_half((double)kx, (double)k2, ph, dt);
+          wv = rk_update(a.stage, e, w0v, a.stage == 0 ? w0v : wcv, rhs, dt);
+        }
+      }
+      st2(bwo, wv, v, so);
+      // the new state replaces P_hat in its slot (read above by this thread only)
+      ldsP[swz(idx)] = wv;
+    };
+#pragma unroll 1
+    for (int pi = g; pi < NPI; pi += FP_GROUPS) {
+      const int q = qof(pi);
+      block(q);
+      block(q + 1);
+    }
+  }
+  __syncthreads();
+
+  // velocity pack of the new state and the inverse column transforms, one field per group
+  constexpr float S = 1.0f / ((float)N * (float)N);
+  float2 X[PTS];
+  {
+    const bool sdo = a.stage == 2;
+    const int f = g;
+    const float cy = f == 1 || sdo ? 1.f : 0.f;
+    const float cx = f == 1 ? (sdo ? -1.f : 0.f) : (sdo ? 1.f : -1.f);
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      if (qout(q)) {
+        X[q] = make_float2(0.f, 0.f);
+        continue;
+      }
+      const int idx = t + q * T;
+      const float2 w = cscale(ldsP[swz(idx)], S);
+      const float fky = (float)wn(idx, N);
+      const float k2 = fkx * fkx + fky * fky;
+      const float inv = k2 > 0.f ? __builtin_amdgcn_rcpf(k2) : 0.f;
+      const float gg = (cy * fky + cx * fkx) * inv;
+      X[q] = f == 0 ? w : cscale(cmuli(w), gg);
+    }
+  }
+  __syncthreads();  // every group has read the new state before group 0 exchanges over it
+  line_fft<N, +1>(X, lds, t, a.tw);
+#pragma unroll
+  for (int q = 0; q < PTS; ++q) st2(bu, X[q], vm, (uint32_t)g * PLANE_B + q * ROWS_B);
+
+  if (a.stage == 2) {
+    // per-step epilogue by the last workgroup of the instance to finish (as in k_cols)
+    __syncthreads();
+    if (tid < 64) {
+      unsigned last = 0u;
+      if (tid == 0) last = atomicAdd(&I->st.cols_done, 1u) == (unsigned)(KC - 1) ? 1u : 0u;
+      last = __shfl(last, 0);
+      if (last) {
+        if (tid < DevState::SPEED_SLOTS)
+          __hip_atomic_store(&I->st.speed_bits[tid], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+          step_epilogue(I->st);
+          __hip_atomic_store(&I->st.cols_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // full-field helpers: plain 2-D FFT (Fft2D.run, fft.js:81-92), state pack / unpack
 // ---------------------------------------------------------------------------------------------
 template <int N, int G, int DIR>
@@ -892,8 +1096,19 @@ struct Impl {
       hipLaunchKernelGGL((k_cols<N, CW, COLS_STEP, false>), grid, dim3(CW * L::T), 0, s->stream,
                          a);
   }
+  static void launch_cols_fp(Solver* s, const ColArgs& a, bool force) {
+    if constexpr (ColFp<N>::on) {
+      constexpr int G8 = (Ge::KC + 7) / 8 * 8;
+      const dim3 grid(G8, s->batch), block(FP_GROUPS * L::T);
+      if (force)
+        hipLaunchKernelGGL((k_cols_fp<N, true>), grid, block, 0, s->stream, a);
+      else
+        hipLaunchKernelGGL((k_cols_fp<N, false>), grid, block, 0, s->stream, a);
+    }
+  }
   template <int I = 0>
   static void launch_cols_width(Solver* s, const ColArgs& a, bool force, int w) {
+    if (ColFp<N>::on && w == FP_WIDTH) return launch_cols_fp(s, a, force);
     if constexpr (I < ColWidths<N>::n) {
       constexpr int CW = ColWidths<N>::w[I];
       if (w == CW) return launch_cols<CW>(s, a, force);
@@ -909,14 +1124,20 @@ struct Impl {
     a.force = force ? 1 : 0;
     a.sd = sd ? 1 : 0;
     prof_begin(s, S2T_PROF_COLS);
-    launch_cols_width(s, a, force, s->cols_c > 0 ? s->cols_c : C);
+    int w0[MAX_COL_WIDTHS];
+    col_widths(w0);
+    launch_cols_width(s, a, force, s->cols_c != 0 ? s->cols_c : w0[0]);
     check_launch("k_cols");
     prof_end(s);
   }
 
+  // the compiled widths, the default first: the field-parallel kernel where it is compiled,
+  // then the column-group widths
   static int col_widths(int* out) {
-    for (int i = 0; i < ColWidths<N>::n; ++i) out[i] = ColWidths<N>::w[i];
-    return ColWidths<N>::n;
+    int k = 0;
+    if (ColFp<N>::on) out[k++] = FP_WIDTH;
+    for (int i = 0; i < ColWidths<N>::n; ++i) out[k++] = ColWidths<N>::w[i];
+    return k;
   }
 
   // velocity prime (encodePrime, solver.js:475-486) of instances i0 .. i0+ni-1 from state[phase]

@@ -67,7 +67,7 @@ def test_detector_sees_the_pattern():
 @pytest.mark.parametrize("src", ["step_kernels.hip", "aux_kernels.hip"])
 def test_no_write_right_after_wide_store(tmp_path, src):
     out = tmp_path / (src + ".s")
-    flags = [f for f in build.HIP_FLAGS if f != "-fPIC"]
+    flags = [f for f in build.HIP_FLAGS if f != "-fPIC"] + build.FILE_FLAGS.get(src, [])
     proc = subprocess.run([build.HIPCC, *flags, "--cuda-device-only", "-S",
                            os.path.join(build.CSRC, src), "-o", str(out)],
                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

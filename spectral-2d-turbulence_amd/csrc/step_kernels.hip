@@ -18,9 +18,10 @@
 //                          column FFTs of omega_hat, u_hat, v_hat -> omega', u', v'.
 //
 // Both kernels stage each line in LDS only for the Stockham exchanges of line_fft. The mixed
-// fields between them are 2-row tiles (Geo::tile): the row kernel reads and writes a row pair as
-// one contiguous run of 16-byte elements; a column group reads and writes 32-byte runs per row
-// pair, and neighbouring groups on one XCD share the cache lines through L2.
+// fields between them are 4-row tiles (Geo::tile): the row kernel reads and writes a row pair as
+// 16-byte elements at a stride of 32 bytes, the two pairs of a tile running back to back on one
+// XCD; a column reads and writes one 32-byte run per tile of 4 rows, and neighbouring column
+// groups on one XCD share the cache lines through L2.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -113,12 +114,19 @@ struct ColWidths {
   static constexpr int n = 1;
   static constexpr int w[4] = {ColCfg<N>::C, 0, 0, 0};
 };
-// sizes with the field-parallel column kernel (k_cols_fp), which is then their default: one
-// instance steps in 54 / 100 us at 512^2 / 1024^2 against 64 / 122 us with the best column-group
-// width; at 2048^2 (206 against 167 us) the column groups already keep the SIMDs busy
+// sizes with the field-parallel column kernel (k_cols_fp), which is then their default, and its
+// number of thread groups per column. One instance steps in 54 / 100 us at 512^2 / 1024^2 with
+// three groups against 64 / 122 us with the best column-group width (two groups: k_cols 12.1 /
+// 22.9 us per launch against 11.5 / 19.4 with three). At 2048^2 two groups take 30.9 us per
+// launch against 33.3 us for column groups of one column (three groups: 38.2 us). At 4096^2 the
+// column groups of two columns are the fastest (104 us; field-parallel 128 us with two groups,
+// 140 us with three): the field-parallel form runs one column per workgroup, 2 columns per CU at
+// a time against 4.
 template <int N>
 struct ColFp {
-  static constexpr bool on = N == 512 || N == 1024;
+  static constexpr int groups = N == 512 || N == 1024 ? 3 : N == 2048 ? 2 : 0;
+  static constexpr bool on = groups > 0;
+  static constexpr int width = -groups;  // its code among the column-group widths
 };
 template <> struct ColWidths<2048> { static constexpr int n = 3; static constexpr int w[4] = {1, 2, 4, 0}; };
 template <> struct ColWidths<1024> { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
@@ -481,7 +489,12 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   const Buf bw0 = make_buf(a.w0 + inst * a.s_state, STATE_B);
   const Buf bwc = make_buf(a.wcur + inst * a.s_state, STATE_B);
   // state leaving the stage (the right-hand side in COLS_RHS); P_hat is parked here first, each
-  // thread at the indices it later overwrites with the new state
+  // thread at the indices it later overwrites with the new state. The park costs a write and a
+  // read of the in-band P_hat (60 MB of a launch's 380 MB at 4096^2); the ways around it measured
+  // slower there: P_hat held in registers across the Q' transform needs 170 VGPRs (one
+  // workgroup per CU) or spills 34-56 at 128, a second LDS line per column leaves room for one
+  // workgroup per CU, and the field-parallel kernel (P' and Q' side by side, both results in LDS)
+  // runs one column per workgroup (128 us per launch against 104).
   const Buf bwo = MODE == COLS_RHS ? make_buf(a.rhs_out, STATE_B)
                                    : make_buf(a.wout + inst * a.s_state, STATE_B);
   const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
@@ -724,20 +737,18 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// column kernel, field-parallel form (grids up to 2048^2, where one thread group per column
-// leaves most SIMDs idle: 342 one-wave workgroups at 1024^2)
+// column kernel, field-parallel form (grids of 512^2 to 2048^2, where one thread group per
+// column leaves SIMDs idle: 342 one-wave workgroups at 1024^2)
 // ---------------------------------------------------------------------------------------------
-// One column per workgroup and three thread groups of T threads side by side: groups 0 and 1
-// transform P' and Q' (group 2 the pointer forcing at stage 0, else an idle transform that keeps
-// the barriers in step); P_hat and Q_hat both stay in LDS (no round trip through the output state);
-// the per-mode loop is split over the three groups by pairs of row blocks; then groups 0, 1, 2
-// transform omega', u', v' (s', d' after stage 2) side by side. Every value goes through the same
-// device code as in k_cols, so the state is the same bit for bit.
-constexpr int FP_GROUPS = 3;
-constexpr int FP_WIDTH = -FP_GROUPS;  // its code among the column-group widths
-
-template <int N, bool FORCE>
-__global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
+// One column per workgroup and GR = 2 or 3 thread groups of T threads side by side: groups 0 and 1
+// transform P' and Q' (with three groups, group 2 transforms the pointer forcing at stage 0, else
+// zeros, which keeps the barriers in step; with two, the force transform runs first, group 1
+// transforming zeros); P_hat and Q_hat both stay in LDS (no round trip through the output state);
+// the per-mode loop is split over the groups by pairs of row blocks; then the inverse transforms
+// of omega', u', v' (s', d' after stage 2) are dealt to the groups. Every value goes through the
+// same device code as in k_cols, so the state is the same bit for bit.
+template <int N, int GR, bool FORCE>
+__global__ void __launch_bounds__(GR * Line<N>::T) k_cols_fp(ColArgs a) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC, NJ = Ge::NJ;
@@ -746,7 +757,8 @@ __global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
   constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), STATE_B = (uint32_t)(Ge::STATE * 8);
   constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;
   static_assert(T % 64 == 0, "thread groups of whole waves");
-  __shared__ float2 lds_all[FP_GROUPS * LSTRIDE];
+  static_assert(GR == 2 || GR == 3, "two or three thread groups");
+  __shared__ float2 lds_all[GR * LSTRIDE];
 
   const int bid = blockIdx.x;
   const int kx = (bid % 8) * (G8 / 8) + bid / 8;  // XCD-aware: a contiguous range of columns per XCD
@@ -802,9 +814,23 @@ __global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
 
   // forward column transforms side by side: P' (group 0), Q' (group 1), F' (group 2, stage 0 with
   // pointer forcing; otherwise group 2 transforms zeros, so that every group meets the same
-  // barriers)
+  // barriers). Two groups transform F' first: group 0 F', group 1 zeros.
+  const bool force_now = FORCE && a.stage == 0;
+  if (GR == 2 && force_now) {
+    float2 X[PTS];
+#pragma unroll
+    for (int i = 0; i < PTS; ++i) {
+      const int q = dft_load_order<PTS>(i);
+      X[q] = ld2(bf, g == 0 ? vm : BUF_OOB, q * ROWS_B);
+    }
+    line_fft<N, -1>(X, lds, t, a.tw);
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q)
+        if (inband(q)) st2(bfk, X[q], vs, soff_s(q));
+    }
+  }
   {
-    const bool force_now = FORCE && a.stage == 0;
     const Buf src = g == 2 ? bf : bz;
     const uint32_t base = g == 1 ? PLANE_B : 0u;
     const uint32_t v = (g < 2 || force_now) ? vm : BUF_OOB;
@@ -865,7 +891,7 @@ __global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
       ldsP[swz(idx)] = wv;
     };
 #pragma unroll 1
-    for (int pi = g; pi < NPI; pi += FP_GROUPS) {
+    for (int pi = g; pi < NPI; pi += GR) {
       const int q = qof(pi);
       block(q);
       block(q + 1);
@@ -873,33 +899,48 @@ __global__ void __launch_bounds__(FP_GROUPS * Line<N>::T) k_cols_fp(ColArgs a) {
   }
   __syncthreads();
 
-  // velocity pack of the new state and the inverse column transforms, one field per group
+  // velocity pack of the new state and the inverse column transforms of the three fields, dealt
+  // to the groups (two groups: group 0 omega' and v', group 1 u' and a transform of zeros, so that
+  // both meet the same barriers)
   constexpr float S = 1.0f / ((float)N * (float)N);
-  float2 X[PTS];
-  {
-    const bool sdo = a.stage == 2;
-    const int f = g;
+  constexpr int ROUNDS = (3 + GR - 1) / GR;
+  const bool sdo = a.stage == 2;
+  // with more than one round the new state is held in registers for all of them
+  float2 W[PTS];
+  if constexpr (ROUNDS > 1) {
+#pragma unroll
+    for (int q = 0; q < PTS; ++q)
+      W[q] = qout(q) ? make_float2(0.f, 0.f) : cscale(ldsP[swz(t + q * T)], S);
+    __syncthreads();  // every group has read the new state before group 0 exchanges over it
+  }
+#pragma unroll 1
+  for (int r = 0; r < ROUNDS; ++r) {
+    const int f = g + r * GR;
+    const int tl = launder(t);
     const float cy = f == 1 || sdo ? 1.f : 0.f;
     const float cx = f == 1 ? (sdo ? -1.f : 0.f) : (sdo ? 1.f : -1.f);
+    float2 X[PTS];
 #pragma unroll
     for (int q = 0; q < PTS; ++q) {
-      if (qout(q)) {
+      if (qout(q) || f > 2) {
         X[q] = make_float2(0.f, 0.f);
         continue;
       }
-      const int idx = t + q * T;
-      const float2 w = cscale(ldsP[swz(idx)], S);
+      const int idx = tl + q * T;
+      const float2 w = ROUNDS > 1 ? W[q] : cscale(ldsP[swz(idx)], S);
       const float fky = (float)wn(idx, N);
       const float k2 = fkx * fkx + fky * fky;
       const float inv = k2 > 0.f ? __builtin_amdgcn_rcpf(k2) : 0.f;
       const float gg = (cy * fky + cx * fkx) * inv;
       X[q] = f == 0 ? w : cscale(cmuli(w), gg);
     }
-  }
-  __syncthreads();  // every group has read the new state before group 0 exchanges over it
-  line_fft<N, +1>(X, lds, t, a.tw);
+    if constexpr (ROUNDS == 1) __syncthreads();  // every group has read the new state
+    line_fft<N, +1>(X, lds, tl, a.tw);
+    if (f <= 2) {
 #pragma unroll
-  for (int q = 0; q < PTS; ++q) st2(bu, X[q], vm, (uint32_t)g * PLANE_B + q * ROWS_B);
+      for (int q = 0; q < PTS; ++q) st2(bu, X[q], vm, (uint32_t)f * PLANE_B + q * ROWS_B);
+    }
+  }
 
   if (a.stage == 2) {
     // per-step epilogue by the last workgroup of the instance to finish (as in k_cols)
@@ -1099,16 +1140,17 @@ struct Impl {
   static void launch_cols_fp(Solver* s, const ColArgs& a, bool force) {
     if constexpr (ColFp<N>::on) {
       constexpr int G8 = (Ge::KC + 7) / 8 * 8;
-      const dim3 grid(G8, s->batch), block(FP_GROUPS * L::T);
+      constexpr int GR = ColFp<N>::groups;
+      const dim3 grid(G8, s->batch), block(GR * L::T);
       if (force)
-        hipLaunchKernelGGL((k_cols_fp<N, true>), grid, block, 0, s->stream, a);
+        hipLaunchKernelGGL((k_cols_fp<N, GR, true>), grid, block, 0, s->stream, a);
       else
-        hipLaunchKernelGGL((k_cols_fp<N, false>), grid, block, 0, s->stream, a);
+        hipLaunchKernelGGL((k_cols_fp<N, GR, false>), grid, block, 0, s->stream, a);
     }
   }
   template <int I = 0>
   static void launch_cols_width(Solver* s, const ColArgs& a, bool force, int w) {
-    if (ColFp<N>::on && w == FP_WIDTH) return launch_cols_fp(s, a, force);
+    if (ColFp<N>::on && w == ColFp<N>::width) return launch_cols_fp(s, a, force);
     if constexpr (I < ColWidths<N>::n) {
       constexpr int CW = ColWidths<N>::w[I];
       if (w == CW) return launch_cols<CW>(s, a, force);
@@ -1135,7 +1177,7 @@ struct Impl {
   // then the column-group widths
   static int col_widths(int* out) {
     int k = 0;
-    if (ColFp<N>::on) out[k++] = FP_WIDTH;
+    if (ColFp<N>::on) out[k++] = ColFp<N>::width;
     for (int i = 0; i < ColWidths<N>::n; ++i) out[k++] = ColWidths<N>::w[i];
     return k;
   }

@@ -69,17 +69,19 @@ def test_bandlimited_spectrum_host_matches_oracle(s2t, n, kmin, kmax):
 
 def test_column_widths_host(s2t):
     """s2t_column_widths (host only): every supported size lists its compiled column-group widths,
-    the default first, each a workgroup of at most 1024 threads (16 values per thread); -3 is the
-    field-parallel column kernel (three thread groups on one column), the default at 512 and
-    1024."""
+    the default first, each a workgroup of at most 1024 threads (16 values per thread); a negative
+    width -g is the field-parallel column kernel (g thread groups on one column), the default with
+    three groups at 512 and 1024 and with two at 2048."""
+    fp = {512: -3, 1024: -3, 2048: -2}
     for n in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
         w = s2t.column_widths(n)
         assert len(w) == len(set(w)) >= 1
         per = n // (8 if n in (32, 64, 512) else 4 if n == 16 else 16)
-        assert all((c >= 1 and c * per <= 1024) or (c == -3 and 3 * per <= 1024) for c in w)
-        assert (w[0] == -3) == (n in (512, 1024))
+        assert all((c >= 1 and c * per <= 1024) or (c in (-2, -3) and -c * per <= 1024) for c in w)
+        assert [c for c in w if c < 0] == ([fp[n]] if n in fp else [])
+        assert (w[0] < 0) == (n in fp)
     assert s2t.column_widths(4096) == [2] and s2t.column_widths(1024)[:2] == [-3, 1]
-    assert len(s2t.column_widths(2048)) > 1
+    assert s2t.column_widths(2048) == [-2, 1, 2, 4]
     with pytest.raises(s2t.S2TError, match="4\\^a"):
         s2t.column_widths(96)
 

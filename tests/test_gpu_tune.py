@@ -30,6 +30,31 @@ def test_every_width_is_bit_identical(s2t, n):
         assert d["stepIndex"] == d0["stepIndex"] == 4 and d["dt"] == d0["dt"]
 
 
+SPLATS = [dict(x=0.3, y=0.6, dx=0.02, dy=-0.01, radius=0.05, amp=400.0, color=(1, 0, 0), dyeAmp=0.5),
+          dict(x=0.95, y=0.02, dx=-0.01, dy=0.03, radius=0.03, amp=250.0, color=(0, 1, 0),
+               dyeAmp=0.2)]
+
+
+@pytest.mark.parametrize("n", [512, 1024, 2048])
+def test_pointer_forcing_every_width(s2t, n):
+    """With pointer forcing (the stage-0 transform of the force field: a thread group of its own in
+    the three-group field-parallel kernel, a transform ahead of P' and Q' in the two-group one)
+    every compiled width gives the same state bit for bit."""
+    res = {}
+    for w in s2t.column_widths(n):
+        with s2t.Solver(n) as s:
+            s.configure(**PHYS)
+            s.seed_spectrum(k0=max(2.0, n / 32), amplitude=5.0, seed=4)
+            s.set_column_width(w)
+            s.set_splats(SPLATS)
+            s.step(3)
+            res[w] = s.read_state()
+    ref = res[s2t.column_widths(n)[0]]
+    assert np.abs(ref).max() > 0
+    for w, got in res.items():
+        assert np.array_equal(got, ref), (n, w)
+
+
 def test_bad_width_rejected(s2t):
     with s2t.Solver(256) as s:
         with pytest.raises(s2t.S2TError, match="not compiled"):

@@ -149,6 +149,15 @@ def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
             el = time.perf_counter() - t0
         else:
             widths, el = default_widths, el_default
+        # grouped launches: six launches per step over every member (s2t_batch_use_grouped)
+        b.use_grouped(True)
+        b.step(warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        el_grouped = time.perf_counter() - t0
+        b.use_grouped(False)
         bad = [i for i in range(len(sizes)) if not np.isfinite(b.diagnostics(i)["energy"])]
     if bad:
         raise RuntimeError("non-finite ensemble members %s" % bad)
@@ -176,6 +185,9 @@ def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
                        "column_widths": {str(k): v for k, v in sorted(widths.items())}},
             "default_widths": {"value": round(pts * steps / el_default / 1e6, 2),
                                "widths": {str(k): v for k, v in sorted(default_widths.items())}},
+            "grouped": {"value": round(pts * steps / el_grouped / 1e6, 2),
+                        "ms_per_step": round(el_grouped * 1e3 / steps, 4),
+                        "launches_per_step": 6},
             "plain_loop": {"value": round(plain_rate, 2), "steps_per_member": plain_steps,
                            "seconds": round(el_plain, 3)},
             "speedup_vs_plain_loop": round(batch_rate / plain_rate, 3)}
@@ -642,7 +654,7 @@ def main(argv=None):
         # loop over the plain single-instance path on the same GPU
         ens = bench_ensemble(s2t, 100, 10, 20, tune=False)
         out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "plain_loop",
-                                               "speedup_vs_plain_loop")}
+                                               "speedup_vs_plain_loop", "grouped")}
         out["ensemble"]["workload"] = ens["config"]["workload"]
         srv = bench_serve(s2t, [10.0, 30.0, 60.0], 80)
         out["serve"] = {"saturation_mpts_steps_per_s": srv["saturation_mpts_steps_per_s"],

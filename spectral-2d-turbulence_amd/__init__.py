@@ -41,7 +41,7 @@ EXPORTS = [
     "s2t_column_widths", "s2t_set_column_width", "s2t_autotune",
     "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
     "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
-    "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_step",
+    "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_use_grouped", "s2t_batch_step",
     "s2t_batch_sync", "s2t_batch_autotune", "s2t_batch_column_width",
     "s2t_server_create", "s2t_server_destroy", "s2t_server_submit", "s2t_server_poll",
     "s2t_server_read_state", "s2t_server_cancel", "s2t_server_set_callback", "s2t_server_drain",
@@ -162,6 +162,7 @@ def load(path=None):
         "s2t_batch_read_state": (ip, [vp, ip, fp]),
         "s2t_batch_diagnostics": (ip, [vp, ip, ctypes.POINTER(Diag), fp, fp]),
         "s2t_batch_use_graphs": (ip, [vp, ip]),
+        "s2t_batch_use_grouped": (ip, [vp, ip]),
         "s2t_batch_step": (ip, [vp, ip]),
         "s2t_batch_sync": (ip, [vp]),
         "s2t_batch_autotune": (ip, [vp, ip]),
@@ -598,6 +599,10 @@ class Batch:
 
     def use_graphs(self, enable):
         _check(self.lib, self.lib.s2t_batch_use_graphs(self.h, 1 if enable else 0))
+
+    def use_grouped(self, enable):
+        """s2t_batch_use_grouped: step the 256/512/1024 classes by grouped launches."""
+        _check(self.lib, self.lib.s2t_batch_use_grouped(self.h, 1 if enable else 0))
 
     def autotune(self, steps=5):
         """s2t_batch_autotune: column-group width per size class; returns {size: width}."""

@@ -6,6 +6,9 @@
 // class branches fork from, and join back to, the batch's main stream, so classes run
 // concurrently and a replay needs no host work. dt, stepIndex and the forcing phases live in the
 // per-member device scalars, so a replayed graph needs no parameter updates.
+// Grouped mode: the classes of sizes 256, 512 and 1024 are stepped together by grouped launches
+// (one launch per row or column pass covering every instance of those classes, on the main
+// stream), the other classes on their branches as before.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +31,7 @@ struct Batch {
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
   bool use_graphs = true;
+  bool grouped = false;
   bool captured = false;
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
@@ -60,9 +64,35 @@ static void batch_quiesce(Batch* b) {
   for (Solver* s : b->cls) sync(s);
 }
 
-// one step of every class, each on its own stream
+// the classes stepped by grouped launches (grouped mode: the grouped sizes), and the others
+static void split_classes(const Batch* b, std::vector<Solver*>& grp, std::vector<Solver*>& rest) {
+  grp.clear();
+  rest.clear();
+  for (Solver* s : b->cls) {
+    if (b->grouped && group_size(s->n) && grp.size() < 4)
+      grp.push_back(s);
+    else
+      rest.push_back(s);
+  }
+  // the largest size first in the work list: its longer workgroups start first and the short
+  // ones of the small sizes fill the tail of the launch
+  std::sort(grp.begin(), grp.end(), [](const Solver* a, const Solver* b) { return a->n > b->n; });
+}
+
+// one step of every class, each on its own stream (grouped classes: on main, after their own
+// streams have drained, and finished before returning)
 static void step_classes(Batch* b) {
-  for (Solver* s : b->cls) step_once(s, false);
+  std::vector<Solver*> grp, rest;
+  split_classes(b, grp, rest);
+  for (Solver* s : rest) step_once(s, false);
+  if (!grp.empty()) {
+    for (Solver* s : grp) {
+      ensure_primed(s);
+      sync(s);
+    }
+    group_step(grp.data(), (int)grp.size(), b->main);
+    hipchk(hipStreamSynchronize(b->main), "hipStreamSynchronize");
+  }
 }
 
 static void capture(Batch* b) {
@@ -72,19 +102,23 @@ static void capture(Batch* b) {
     sync(s);
   }
   const int phase0 = b->cls[0]->phase;
+  std::vector<Solver*> grp, rest;
+  split_classes(b, grp, rest);
   for (int k = 0; k < 3; ++k) {
     const int p = (phase0 + k) % 3;
     for (Solver* s : b->cls) s->phase = p;
     hipchk(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
       hipchk(hipEventRecord(b->fork, b->main), "hipEventRecord");
-      for (size_t c = 0; c < b->cls.size(); ++c) {
-        Solver* s = b->cls[c];
+      for (size_t c = 0; c < rest.size(); ++c) {
+        Solver* s = rest[c];
         hipchk(hipStreamWaitEvent(s->stream, b->fork, 0), "hipStreamWaitEvent");
         step_once(s, false);
         hipchk(hipEventRecord(b->join[c], s->stream), "hipEventRecord");
-        hipchk(hipStreamWaitEvent(b->main, b->join[c], 0), "hipStreamWaitEvent");
       }
+      if (!grp.empty()) group_step(grp.data(), (int)grp.size(), b->main);
+      for (size_t c = 0; c < rest.size(); ++c)
+        hipchk(hipStreamWaitEvent(b->main, b->join[c], 0), "hipStreamWaitEvent");
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(b->main, &g);
@@ -264,6 +298,16 @@ int s2t_batch_use_graphs(s2t_batch* h, int enable) {
   batch_quiesce(b);
   b->use_graphs = enable != 0;
   if (!b->use_graphs) drop_graphs(b);
+  S2T_API_END
+}
+
+int s2t_batch_use_grouped(s2t_batch* h, int enable) {
+  Batch* b = reinterpret_cast<Batch*>(h);
+  if (!b) return api_bad("s2t_batch_use_grouped: null batch");
+  S2T_API_BEGIN
+  batch_quiesce(b);
+  b->grouped = enable != 0;
+  drop_graphs(b);
   S2T_API_END
 }
 

@@ -37,6 +37,12 @@ constexpr int MAX_COL_WIDTHS = 6;
 
 const Ops* ops_for(int n);
 
+// Grouped launches (step_kernels.hip): one step of up to four size classes of sizes 256, 512 or
+// 1024 (no pointer forcing, no dye, one buffer phase, primed) in six launches, each covering every
+// instance of every class.
+bool group_size(int n);
+void group_step(Solver* const* cls, int count, hipStream_t stream);
+
 enum ColsMode { COLS_STEP = 0, COLS_PRIME = 1, COLS_RHS = 2 };
 
 struct Solver {

@@ -281,21 +281,29 @@ __device__ __forceinline__ void row_load(float2 (&x)[Line<N>::PTS], Buf mu, int 
 // VGPRs, or s and d each formed from both planes, a second read of u' and v'.)
 enum RowKind { ROWS_SINGLE = 0, ROWS_SINGLE_FORCE = 1, ROWS_PAIRED = 2, ROWS_PAIRED_SD = 3 };
 
+// LDS of the row kernel: G exchange lines; one real row of each pair (v(y0+1), or omega(y0+1) in
+// ROWS_PAIRED), parked by each thread at its own indices while the other fields are transformed
+// (holding it in registers instead would cost 16 VGPRs across two transforms); the wave maxima
+template <int N, int G>
+struct RowSmem {
+  float2 lds[G * Line<N>::LDS];
+  float vpark[G * N];
+  float wmax[G * Line<N>::T / 64];
+};
+
+// One workgroup of the row kernel: workgroup bx (of the instance's grid of row blocks) of
+// instance iy. Launched on its own by k_rows, or as an item of the grouped launch k_rows_group.
 template <int N, int G, int KIND>
-__global__ void __launch_bounds__(G * Line<N>::T,
-                                  N == 4096 && KIND != ROWS_SINGLE_FORCE ? 3 : 1) k_rows(RowArgs a) {
+__device__ __forceinline__ void rows_body(const RowArgs& a, int bx, int iy, RowSmem<N, G>& sm) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS;
   constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8);
-  __shared__ float2 lds_all[G * L::LDS];
-  // one real row of the pair (v(y0+1), or omega(y0+1) in ROWS_PAIRED), parked by each thread at
-  // its own indices while the other fields are transformed (holding it in registers instead would
-  // cost 16 VGPRs across two transforms)
-  __shared__ float vpark_all[G * N];
-  Inst* I = a.inst + blockIdx.y;
+  float2* lds_all = sm.lds;
+  float* vpark_all = sm.vpark;
+  Inst* I = a.inst + iy;
   if (!I->st.active) return;  // uniform over the block
-  const size_t inst = blockIdx.y;
+  const size_t inst = iy;
   constexpr bool FORCE = KIND == ROWS_SINGLE_FORCE;
   constexpr bool do_force = FORCE;
   const bool do_vel = a.flags & ROW_VEL;
@@ -308,8 +316,8 @@ __global__ void __launch_bounds__(G * Line<N>::T,
   constexpr int PAIRS = RowCfg<N>::PAIRS;
   // XCD-aware mapping: consecutive workgroup ids go round-robin over the 8 XCDs; each XCD gets a
   // contiguous range of row blocks, so the row pairs of one tile run back to back on one XCD
-  constexpr int NB = RowCfg<N>::NB, NB8 = (NB + 7) / 8 * 8;
-  const int lb = (int)(blockIdx.x % 8) * (NB8 / 8) + (int)(blockIdx.x / 8);
+  constexpr int NB = (N / (2 * PAIRS) + G - 1) / G, NB8 = (NB + 7) / 8 * 8;
+  const int lb = (bx % 8) * (NB8 / 8) + bx / 8;
   if (lb >= NB) return;  // uniform over the block
   const int yb = lb * G + g;
   const bool live = yb < N / (2 * PAIRS);
@@ -442,7 +450,7 @@ __global__ void __launch_bounds__(G * Line<N>::T,
 
   if (a.flags & ROW_SPEED) {
     // wave max, workgroup max through LDS, one atomic per workgroup into the block's slot
-    __shared__ float wmax[G * L::T / 64];
+    float* wmax = sm.wmax;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmaxf(smax, __shfl_xor(smax, off));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = smax;
@@ -450,18 +458,33 @@ __global__ void __launch_bounds__(G * Line<N>::T,
     if (threadIdx.x == 0) {
       float m = wmax[0];
       for (int w = 1; w < G * L::T / 64; ++w) m = fmaxf(m, wmax[w]);
-      atomicMax(&I->st.speed_bits[blockIdx.x % DevState::SPEED_SLOTS], __float_as_uint(m));
+      atomicMax(&I->st.speed_bits[bx % DevState::SPEED_SLOTS], __float_as_uint(m));
     }
     if (bad) atomicOr(&I->st.nonfinite, 1u);
   }
+}
+
+template <int N, int G, int KIND>
+__global__ void __launch_bounds__(G * Line<N>::T,
+                                  N == 4096 && KIND != ROWS_SINGLE_FORCE ? 3 : 1) k_rows(RowArgs a) {
+  __shared__ RowSmem<N, G> sm;
+  rows_body<N, G, KIND>(a, (int)blockIdx.x, (int)blockIdx.y, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
 // column kernel
 // ---------------------------------------------------------------------------------------------
 
+template <int N, int C>
+struct ColSmem {
+  float2 lds[C * ColCfg<N>::LSTRIDE];
+  float speed;
+};
+
+// One workgroup of the column kernel: workgroup bx (of the instance's grid of column groups) of
+// instance iy. Launched on its own by k_cols, or as an item of the grouped launch k_cols_group.
 template <int N, int C, int MODE, bool FORCE>
-__global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
+__device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColSmem<N, C>& sm) {
   using L = Line<N>;
   using Ge = Geo<N>;
   constexpr int T = L::T, PTS = L::PTS, K = Ge::K, KC = Ge::KC, NJ = Ge::NJ;
@@ -470,20 +493,20 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
   constexpr int G8 = (GROUPS + 7) / 8 * 8;
   constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8), STATE_B = (uint32_t)(Ge::STATE * 8);
   constexpr uint32_t ROWS_B = (uint32_t)T * Ge::KP * 8;  // row block q -> q + 1 (T / 2 row pairs)
-  __shared__ float2 lds_all[C * LSTRIDE];
+  float2* lds_all = sm.lds;
 
   // XCD-aware mapping: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD
   // a contiguous range of column groups (neighbouring groups share L2 lines of the planes).
-  const int bid = blockIdx.x;
+  const int bid = bx;
   const int cg = (bid % 8) * (G8 / 8) + bid / 8;
   if (cg >= GROUPS) return;
-  Inst* I = a.inst + blockIdx.y;
+  Inst* I = a.inst + iy;
   if (!I->st.active) return;  // uniform over the block
   // a copy taken before the kernel's first store: the loads are known unclobbered, so they are
   // scalar loads and the parameters stay in SGPRs (a reference would be re-read from memory after
   // every buffer store, each time with a wait for all loads in flight)
   const Phys ph = I->ph;
-  const size_t inst = blockIdx.y;
+  const size_t inst = iy;
   const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
   const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
   const Buf bw0 = make_buf(a.w0 + inst * a.s_state, STATE_B);
@@ -533,15 +556,14 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       step = I->st.step;
       if (a.stage == 0) {
         // the step's speed maximum: max over the row kernel's slots
-        __shared__ float speed_s;
         if (tid < 64) {
           float v = __uint_as_float(I->st.speed_bits[tid % DevState::SPEED_SLOTS]);
 #pragma unroll
           for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-          if (tid == 0) speed_s = v;
+          if (tid == 0) sm.speed = v;
         }
         __syncthreads();
-        const float speed = speed_s;
+        const float speed = sm.speed;
         dt = ph.fixed_dt > 0.0 ? ph.fixed_dt : cfl_dt(speed, ph);
         if (bid == 0 && tid == 0) {
           I->st.dt = dt;
@@ -734,6 +756,12 @@ __global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
       }
     }
   }
+}
+
+template <int N, int C, int MODE, bool FORCE>
+__global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
+  __shared__ ColSmem<N, C> sm;
+  cols_body<N, C, MODE, FORCE>(a, (int)blockIdx.x, (int)blockIdx.y, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1258,6 +1286,174 @@ struct Impl {
     return &o;
   }
 };
+
+// ---------------------------------------------------------------------------------------------
+// grouped launches: one launch per pass over the instances of several grid sizes
+// ---------------------------------------------------------------------------------------------
+// The workgroups of a launch form a list of (size class, instance, block) items: class c owns
+// workgroups first[c] .. first[c] + nb[c] * instances - 1, instance-major, and each workgroup runs
+// the row or column body of its class's grid size with 256 threads (several lines or columns of
+// the smaller sizes per workgroup). Every class's range starts at a multiple of 8, so the XCD of a
+// workgroup is that of its block index within the instance, as in the per-class launches.
+constexpr int GROUP_THREADS = 256;
+constexpr int GROUP_MAX = 4;
+
+template <int N>
+struct GroupCfg {
+  static constexpr int G = GROUP_THREADS / Line<N>::T;  // row blocks per row workgroup
+  static constexpr int C = GROUP_THREADS / Line<N>::T;  // columns per column workgroup
+  static constexpr int ROW_NB8 = ((N / 2 + G - 1) / G + 7) / 8 * 8;
+  static constexpr int COL_NB8 = ((Geo<N>::KC + C - 1) / C + 7) / 8 * 8;
+};
+static_assert(GroupCfg<256>::C == 16 && GroupCfg<512>::C == 4 && GroupCfg<1024>::C == 4,
+              "grouped column workgroups use compiled column-group widths");
+
+struct GroupClass {
+  float2* mu;                      // omega', u', v' tiles
+  float2* mz;                      // P', Q' tiles
+  float2* w0, *wcur, *wout;        // states of the stage (column launches)
+  const float2* tw;
+  Inst* inst;
+  int n, first, nb;  // grid size; first workgroup, workgroups per instance in this launch
+};
+struct GroupArgs {
+  GroupClass c[GROUP_MAX];
+  int count, stage;
+};
+
+union RowGroupSmem {
+  RowSmem<256, GroupCfg<256>::G> n256;
+  RowSmem<512, GroupCfg<512>::G> n512;
+  RowSmem<1024, GroupCfg<1024>::G> n1024;
+};
+union ColGroupSmem {
+  ColSmem<256, GroupCfg<256>::C> n256;
+  ColSmem<512, GroupCfg<512>::C> n512;
+  ColSmem<1024, GroupCfg<1024>::C> n1024;
+};
+
+// the class of workgroup b, and its block and instance within the class
+__device__ __forceinline__ const GroupClass& group_item(const GroupArgs& g, int& bx, int& iy) {
+  const int b = (int)blockIdx.x;
+  int c = 0;
+  while (c + 1 < g.count && b >= g.c[c + 1].first) ++c;
+  const GroupClass& k = g.c[c];
+  const int local = b - k.first;
+  iy = local / k.nb;
+  bx = local - iy * k.nb;
+  return k;
+}
+
+template <int N, int KIND>
+__device__ __forceinline__ void group_rows(const GroupClass& k, int stage, int bx, int iy,
+                                           RowSmem<N, GroupCfg<N>::G>& sm) {
+  RowArgs a{};
+  a.mu = k.mu;
+  a.mz = k.mz;
+  a.tw = k.tw;
+  a.inst = k.inst;
+  a.s_mu = 3 * Geo<N>::PLANE;
+  a.s_mz = 2 * Geo<N>::PLANE;
+  a.stage = stage;
+  a.flags = KIND == ROWS_PAIRED_SD ? ROW_SPEED : 0;
+  rows_body<N, GroupCfg<N>::G, KIND>(a, bx, iy, sm);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(GROUP_THREADS) k_rows_group(GroupArgs g) {
+  __shared__ RowGroupSmem sm;
+  int bx, iy;
+  const GroupClass& k = group_item(g, bx, iy);
+  switch (k.n) {
+    case 256: group_rows<256, KIND>(k, g.stage, bx, iy, sm.n256); break;
+    case 512: group_rows<512, KIND>(k, g.stage, bx, iy, sm.n512); break;
+    case 1024: group_rows<1024, KIND>(k, g.stage, bx, iy, sm.n1024); break;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void group_cols(const GroupClass& k, int stage, int bx, int iy,
+                                           ColSmem<N, GroupCfg<N>::C>& sm) {
+  ColArgs a{};
+  a.mz = k.mz;
+  a.mu = k.mu;
+  a.w0 = k.w0;
+  a.wcur = k.wcur;
+  a.wout = k.wout;
+  a.tw = k.tw;
+  a.inst = k.inst;
+  a.s_mz = 2 * Geo<N>::PLANE;
+  a.s_mu = 3 * Geo<N>::PLANE;
+  a.s_state = Geo<N>::STATE;
+  a.stage = stage;
+  a.sd = stage == 0 ? 1 : 0;  // the stage-0 row launch writes the sum / difference form
+  cols_body<N, GroupCfg<N>::C, COLS_STEP, false>(a, bx, iy, sm);
+}
+
+__global__ void __launch_bounds__(GROUP_THREADS) k_cols_group(GroupArgs g) {
+  __shared__ ColGroupSmem sm;
+  int bx, iy;
+  const GroupClass& k = group_item(g, bx, iy);
+  switch (k.n) {
+    case 256: group_cols<256>(k, g.stage, bx, iy, sm.n256); break;
+    case 512: group_cols<512>(k, g.stage, bx, iy, sm.n512); break;
+    case 1024: group_cols<1024>(k, g.stage, bx, iy, sm.n1024); break;
+  }
+}
+
+bool group_size(int n) { return n == 256 || n == 512 || n == 1024; }
+
+// One step of the classes (each of a grouped size, with no pointer forcing and no dye, all at the
+// same buffer phase and primed) in six launches on `stream`.
+void group_step(Solver* const* cls, int count, hipStream_t stream) {
+  if (count < 1 || count > GROUP_MAX) throw Fail(S2T_EINVAL, "grouped step: 1 to 4 size classes");
+  auto nb8 = [](int n, bool rows) {
+    switch (n) {
+      case 256: return rows ? GroupCfg<256>::ROW_NB8 : GroupCfg<256>::COL_NB8;
+      case 512: return rows ? GroupCfg<512>::ROW_NB8 : GroupCfg<512>::COL_NB8;
+      case 1024: return rows ? GroupCfg<1024>::ROW_NB8 : GroupCfg<1024>::COL_NB8;
+    }
+    throw Fail(S2T_EINVAL, "grouped step: unsupported grid size " + std::to_string(n));
+  };
+  for (int stage = 0; stage < 3; ++stage) {
+    for (int rows = 1; rows >= 0; --rows) {
+      GroupArgs g{};
+      g.count = count;
+      g.stage = stage;
+      int total = 0;
+      for (int c = 0; c < count; ++c) {
+        const Solver* s = cls[c];
+        if (s->splats.count > 0) throw Fail(S2T_EINVAL, "grouped step: no pointer forcing");
+        GroupClass& k = g.c[c];
+        const int p = s->phase;
+        k.mu = s->mixed_u;
+        k.mz = s->mixed_z;
+        k.w0 = s->state[p];
+        k.wcur = s->state[(p + stage) % 3];
+        k.wout = s->state[(p + (stage == 1 ? 2 : 1)) % 3];
+        k.tw = s->tw;
+        k.inst = s->inst;
+        k.n = s->n;
+        k.nb = nb8(s->n, rows != 0);
+        k.first = total;
+        total += k.nb * s->batch;
+      }
+      if (rows) {
+        if (stage == 0)
+          hipLaunchKernelGGL((k_rows_group<ROWS_PAIRED_SD>), dim3(total), dim3(GROUP_THREADS), 0,
+                             stream, g);
+        else
+          hipLaunchKernelGGL((k_rows_group<ROWS_PAIRED>), dim3(total), dim3(GROUP_THREADS), 0,
+                             stream, g);
+        check_launch("k_rows_group");
+      } else {
+        hipLaunchKernelGGL(k_cols_group, dim3(total), dim3(GROUP_THREADS), 0, stream, g);
+        check_launch("k_cols_group");
+      }
+    }
+  }
+  for (int c = 0; c < count; ++c) cls[c]->phase = (cls[c]->phase + 1) % 3;
+}
 
 const Ops* ops_for(int n) {
   switch (n) {

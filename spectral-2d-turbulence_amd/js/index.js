@@ -544,6 +544,11 @@ class Batch {
     addon().batchUseGraphs(this.handle, !!on);
   }
 
+  // grouped launches for the 256 / 512 / 1024 classes (s2t_batch_use_grouped)
+  useGrouped(on) {
+    addon().batchUseGrouped(this.handle, !!on);
+  }
+
   step(count = 1) {
     addon().batchStep(this.handle, count);
   }

@@ -1000,6 +1000,17 @@ napi_value BatchUseGraphs(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+napi_value BatchUseGrouped(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  s2t_batch* b = batch(env, a.argv[0]);
+  if (!b) return nullptr;
+  bool on = false;
+  napi_get_value_bool(env, a.argv[1], &on);
+  S2T_CALL(env, s2t_batch_use_grouped(b, on ? 1 : 0));
+  return nullptr;
+}
+
 // batchAutotune(h, steps); batchColumnWidth(h, i) -> width
 napi_value BatchAutotune(napi_env env, napi_callback_info info) {
   Args a;
@@ -1359,7 +1370,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"batchConfigure", BatchConfigure}, {"batchSeedSpectrum", BatchSeedSpectrum},
              {"batchSeedPhysical", BatchSeedPhysical}, {"batchWriteState", BatchWriteState},
              {"batchReadState", BatchReadState}, {"batchDiagnostics", BatchDiagnostics},
-             {"batchUseGraphs", BatchUseGraphs}, {"batchStep", BatchStep},
+             {"batchUseGraphs", BatchUseGraphs}, {"batchUseGrouped", BatchUseGrouped},
+             {"batchStep", BatchStep},
              {"batchSync", BatchSync}, {"batchAutotune", BatchAutotune},
              {"batchColumnWidth", BatchColumnWidth}, {"autotune", Autotune},
              {"setClock", SetClock},    {"prime", Prime},

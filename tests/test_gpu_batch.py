@@ -48,6 +48,34 @@ def test_batch_members_match_plain(s2t, graphs):
     assert np.array_equal(mid, want_mid)
 
 
+GROUPED_SIZES = [256, 512, 256, 1024, 128, 512, 256, 2048]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_grouped_members_match_plain(s2t, graphs):
+    """Grouped launches (s2t_batch_use_grouped): the 256, 512 and 1024 classes step together in six
+    launches per step, the 128 and 2048 classes on their own; every member is bit-identical to its
+    plain solver, eager or replayed, also when the mode is switched between steps."""
+    sizes = GROUPED_SIZES
+    with s2t.Batch(sizes) as b:
+        b.use_graphs(graphs)
+        b.use_grouped(True)
+        for i, n in enumerate(sizes):
+            b.configure(i, **member_params(i))
+            b.seed_spectrum(i, k0=n / 10, amplitude=6.0, seed=1000 + i)
+        b.step(3)
+        b.use_grouped(False)
+        b.step(2)
+        b.use_grouped(True)
+        b.step(3)
+        got = [b.read_state(i) for i in range(len(sizes))]
+        diags = [b.diagnostics(i) for i in range(len(sizes))]
+    for i, n in enumerate(sizes):
+        want, d = run_plain(s2t, n, member_params(i), 1000 + i, 8)
+        assert np.array_equal(got[i], want), "member %d (%d^2) differs" % (i, n)
+        assert diags[i]["stepIndex"] == 8 and diags[i]["time"] == d["time"]
+
+
 def test_batch_reseed_member_between_replays(s2t):
     """Replacing one member's state re-primes it; the others continue undisturbed."""
     sizes = [128, 128, 128]

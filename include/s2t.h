@@ -372,6 +372,10 @@ int s2t_server_get_stats(s2t_server* s, s2t_server_stats* out);
 int s2t_column_widths(int n, int* widths, int max, int* count);
 /* Use a compiled width for this solver's steps (0: the size's default). */
 int s2t_set_column_width(s2t_solver* h, int width);
+/* Row kernel of this solver's steps: 1 (the default) the field-parallel row kernel where it is
+ * compiled (512 and 1024: three thread groups per row pair), 0 one thread group per row pair.
+ * Both give bit-identical results; other sizes have one row kernel and ignore the choice. */
+int s2t_set_row_kernel(s2t_solver* h, int field_parallel);
 /* Times `steps` steps per compiled width on a scratch solver of the same size and parameters (the
  * solver's own state is not touched) and keeps the fastest: *width, and the milliseconds per step
  * of each width in s2t_column_widths order in ms_per_step[0 .. max-1]. */

@@ -38,7 +38,7 @@ EXPORTS = [
     "s2t_paint_dye", "s2t_dye_size", "s2t_read_dye", "s2t_write_dye", "s2t_write_velocity",
     "s2t_dye_step", "s2t_fft2", "s2t_velocity_pack", "s2t_assemble", "s2t_rhs", "s2t_profile",
     "s2t_profile_kinds", "s2t_profile_sampled", "s2t_profile_read", "s2t_kernel_bytes", "s2t_copy_probe",
-    "s2t_column_widths", "s2t_set_column_width", "s2t_autotune",
+    "s2t_column_widths", "s2t_set_column_width", "s2t_set_row_kernel", "s2t_autotune",
     "s2t_batch_create", "s2t_batch_destroy", "s2t_batch_size", "s2t_batch_configure",
     "s2t_batch_seed_spectrum", "s2t_batch_seed_physical", "s2t_batch_write_state",
     "s2t_batch_read_state", "s2t_batch_diagnostics", "s2t_batch_use_graphs", "s2t_batch_use_grouped", "s2t_batch_step",
@@ -151,6 +151,7 @@ def load(path=None):
         "s2t_copy_probe": (ip, [dp, ip, ctypes.POINTER(dp)]),
         "s2t_column_widths": (ip, [ip, ctypes.POINTER(ip), ip, ctypes.POINTER(ip)]),
         "s2t_set_column_width": (ip, [vp, ip]),
+        "s2t_set_row_kernel": (ip, [vp, ip]),
         "s2t_autotune": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(dp), ip]),
         "s2t_batch_create": (ip, [ctypes.POINTER(ctypes.c_int), ip, ctypes.POINTER(vp)]),
         "s2t_batch_destroy": (ip, [vp]),
@@ -444,6 +445,11 @@ class Solver:
     def set_column_width(self, width):
         """s2t_set_column_width: a compiled column-group width for the steps (0: the default)."""
         _check(self.lib, self.lib.s2t_set_column_width(self.h, int(width)))
+
+    def set_row_kernel(self, field_parallel):
+        """s2t_set_row_kernel: the field-parallel row kernel (True, the default where compiled) or
+        one thread group per row pair (False)."""
+        _check(self.lib, self.lib.s2t_set_row_kernel(self.h, 1 if field_parallel else 0))
 
     def autotune(self, steps=5):
         """s2t_autotune: (chosen width, {width: ms per step}) from a scratch solver."""

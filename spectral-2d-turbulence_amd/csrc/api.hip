@@ -535,6 +535,14 @@ int s2t_set_column_width(s2t_solver* h, int width) {
   S2T_API_END
 }
 
+int s2t_set_row_kernel(s2t_solver* h, int field_parallel) {
+  SOLVER(h);
+  if (field_parallel != 0 && field_parallel != 1) return api_bad("s2t_set_row_kernel: 0 or 1");
+  S2T_API_BEGIN
+  s->rows_fp = field_parallel != 0;
+  S2T_API_END
+}
+
 int s2t_autotune(s2t_solver* h, int steps, int* width, double* ms_per_step, int max) {
   SOLVER(h);
   if (steps < 1) return api_bad("s2t_autotune: steps must be >= 1");

@@ -64,6 +64,7 @@ struct Solver {
   Inst* inst = nullptr;        // per-instance parameters and scalars (batch entries)
   int batch = 1;               // instances sharing this solver's buffers
   int cols_c = 0;              // column-kernel width in a step (0: the default; < 0: field-parallel)
+  bool rows_fp = true;         // field-parallel row kernel where it is compiled (512, 1024)
   double* diag_bins = nullptr; // [2][NBINS] energy and enstrophy per bin, then 2 totals
   double* diag_zonal = nullptr;
   double* seed_sum = nullptr;  // per-column partial sums for fillSpectrum normalisation

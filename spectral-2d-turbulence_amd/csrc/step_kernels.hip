@@ -187,17 +187,23 @@ static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() ==
 // `voff` is the lane's byte offset of (row pair, kx = t). PAIR: the fields are rows 2m and 2m+1
 // of one plane, stored together as one 16-byte element; otherwise they are one row of two planes
 // (uniform byte offsets soff0 / soff1, which include the row's 8-byte slot in the pair).
-template <int N, bool PAIR>
-__device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], float2* lds, int t,
-                                                 bool live, Buf out, uint32_t voff,
-                                                 uint32_t soff0, uint32_t soff1) {
+// The two halves: row_unpack_stage puts the line in LDS (every thread of the workgroup calls it:
+// it holds the barriers), row_unpack_emit stores the unpacked values of one line.
+template <int N>
+__device__ __forceinline__ void row_unpack_stage(float2 (&z)[Line<N>::PTS], float2* lds, int t) {
   using L = Line<N>;
-  using Ge = Geo<N>;
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < L::PTS; ++q)
     if (!ColBand<N>::out(q)) lds[swz(t + q * L::T)] = z[q];  // the others are never read
   __syncthreads();
+}
+
+template <int N, bool PAIR>
+__device__ __forceinline__ void row_unpack_emit(const float2* lds, int t, bool live, Buf out,
+                                                uint32_t voff, uint32_t soff0, uint32_t soff1) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
   if (!live) return;
 #pragma unroll
   for (int q = 0; q < L::PTS; ++q) {
@@ -217,6 +223,14 @@ __device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], floa
       }
     }
   }
+}
+
+template <int N, bool PAIR>
+__device__ __forceinline__ void row_unpack_store(float2 (&z)[Line<N>::PTS], float2* lds, int t,
+                                                 bool live, Buf out, uint32_t voff,
+                                                 uint32_t soff0, uint32_t soff1) {
+  row_unpack_stage<N>(z, lds, t);
+  row_unpack_emit<N, PAIR>(lds, t, live, out, voff, soff0, soff1);
 }
 
 // Inverse row transform input, Hermitian-extended to the full row: thread t, value q holds
@@ -469,6 +483,156 @@ __global__ void __launch_bounds__(G * Line<N>::T,
                                   N == 4096 && KIND != ROWS_SINGLE_FORCE ? 3 : 1) k_rows(RowArgs a) {
   __shared__ RowSmem<N, G> sm;
   rows_body<N, G, KIND>(a, (int)blockIdx.x, (int)blockIdx.y, sm);
+}
+
+// Row kernel, field-parallel form (512^2 and 1024^2, where one 64-thread group per row pair
+// leaves most SIMDs idle: 512 one-wave workgroups at 1024^2). One row pair per workgroup and three
+// thread groups of T threads: the three inverse transforms side by side, then the two forward
+// transforms side by side while the third group transforms the pointer forcing (or zeros, which
+// keeps the barriers in step). Paired kinds: groups 0, 1, 2 take the omega, u, v pairs (omega, s, d
+// at stage 0), omega goes to LDS, groups 1 and 2 form and transform the P' and Q' (S', D') pairs.
+// Single kinds (stage 0 with dye velocity): group 0 takes the d pair, groups 1 and 2 the
+// omega + i s rows y0 and y0 + 1, the d pair goes to LDS, group 1 + r forms the products of row
+// y0 + r. Every value goes through the same device code as in k_rows, so the state is the same bit
+// for bit.
+// Sizes with the field-parallel row kernel (the default there): 7.7 / 14.0 us per launch at
+// 512^2 / 1024^2 against 9.2 / 16.7 us with one group per row pair; at 2048^2 (two waves per row
+// pair already, 67 KB of LDS per three-group workgroup) it took 41.3 us against 24.3 us.
+template <int N>
+struct RowFp {
+  static constexpr bool on = N == 512 || N == 1024;
+};
+constexpr int ROWFP_GROUPS = 3;
+
+template <int N, int KIND>
+__global__ void __launch_bounds__(ROWFP_GROUPS * Line<N>::T) k_rows_fp(RowArgs a) {
+  using L = Line<N>;
+  using Ge = Geo<N>;
+  constexpr int T = L::T, PTS = L::PTS;
+  constexpr uint32_t PLANE_B = (uint32_t)(Ge::PLANE * 8);
+  constexpr bool PAIRED = KIND == ROWS_PAIRED || KIND == ROWS_PAIRED_SD;
+  constexpr bool SD = KIND == ROWS_PAIRED_SD;
+  constexpr bool FORCE = KIND == ROWS_SINGLE_FORCE;
+  static_assert(T % 64 == 0, "thread groups of whole waves");
+  __shared__ float2 lds_all[ROWFP_GROUPS * L::LDS];
+  __shared__ float2 share[N];  // the omega pair (paired kinds) or the d pair (single kinds)
+  __shared__ float wmax[ROWFP_GROUPS * T / 64];
+  Inst* I = a.inst + blockIdx.y;
+  if (!I->st.active) return;  // uniform over the block
+  const size_t inst = blockIdx.y;
+  const bool do_vel = a.flags & ROW_VEL;
+  const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
+  const Buf bz = make_buf(a.mz + inst * a.s_mz, 2 * PLANE_B);
+  const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
+  const Buf bv = make_buf(do_vel ? a.vel + inst * a.s_vel : a.vel,
+                          do_vel ? (uint32_t)((size_t)N * N * 8) : 0u);
+  constexpr int NB = N / 2, NB8 = (NB + 7) / 8 * 8;
+  const int bx = (int)blockIdx.x;
+  const int lb = (bx % 8) * (NB8 / 8) + bx / 8;  // XCD-aware, as in k_rows
+  if (lb >= NB) return;  // uniform over the block
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int y0 = 2 * lb;
+  float2* lds = lds_all + g * L::LDS;
+  const uint32_t vlo = Ge::off(y0, t);
+  const uint32_t vhi = Ge::off(y0, T - t);
+  float smax = 0.f;
+  bool bad = false;
+
+  // inverse transforms, one field per group
+  float2 X[PTS];
+  if constexpr (PAIRED) {
+    row_load<N, true>(X, bu, t, true, vlo, vhi, (uint32_t)g * PLANE_B, 0u);
+  } else {
+    // group 0: d pair; group 1 + r: omega + i s of row y0 + r
+    if (g == 0) {
+      row_load<N, true>(X, bu, t, true, vlo, vhi, 2 * PLANE_B, 0u);
+    } else {
+      const uint32_t r = (uint32_t)(g - 1);
+      row_load<N, false>(X, bu, t, true, vlo, vhi, r * 8u, PLANE_B + r * 8u);
+    }
+  }
+  line_fft<N, +1>(X, lds, t, a.tw);
+  if (g == 0) {
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) share[t + q * T] = X[q];
+  }
+  if constexpr (SD) {
+    if (g > 0) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float m0 = fabsf(X[q].x), m1 = fabsf(X[q].y);
+        bad |= !(m0 <= 3.4e38f) || !(m1 <= 3.4e38f);
+        smax = fmaxf(smax, fmaxf(m0, m1));
+      }
+    }
+  }
+  __syncthreads();
+
+  // products and forward transforms (groups 1, 2); pointer forcing or zeros (group 0)
+  if (g > 0) {
+    if constexpr (PAIRED) {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float2 om = share[t + q * T];
+        X[q] = make_float2(X[q].x * om.x, X[q].y * om.y);
+      }
+    } else {
+      const int r = g - 1;
+      const uint32_t vvel = ((uint32_t)(y0 + r) * N + (uint32_t)t) * 8u;
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float om = X[q].x, sv = X[q].y;
+        const float2 dp = share[t + q * T];
+        const float dv = r == 0 ? dp.x : dp.y;
+        if (a.flags & ROW_SPEED) {
+          const float sp = fmaxf(fabsf(sv), fabsf(dv));  // = |u| + |v|
+          bad |= !(sp <= 3.4e38f);
+          smax = fmaxf(smax, sp);
+        }
+        if (do_vel)
+          st2(bv, make_float2(0.5f * (sv + dv), 0.5f * (sv - dv)), vvel, (uint32_t)(q * T) * 8u);
+        X[q] = make_float2(sv * om, dv * om);
+      }
+    }
+  } else {
+    if constexpr (FORCE) {
+      const float yu0 = ((float)y0 + 0.5f) / (float)N;
+      const float yu1 = ((float)(y0 + 1) + 0.5f) / (float)N;
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) {
+        const float xu = ((float)(t + q * T) + 0.5f) / (float)N;
+        X[q] = make_float2(pointer_curl(xu, yu0, a.splats), pointer_curl(xu, yu1, a.splats));
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PTS; ++q) X[q] = make_float2(0.f, 0.f);
+    }
+  }
+  line_fft<N, -1>(X, lds, t, a.tw);
+  row_unpack_stage<N>(X, lds, t);
+  if (g > 0) {
+    if constexpr (PAIRED) {
+      row_unpack_emit<N, true>(lds, t, true, bz, vlo, (uint32_t)(g - 1) * PLANE_B, 0u);
+    } else {
+      const uint32_t r = (uint32_t)(g - 1);
+      row_unpack_emit<N, false>(lds, t, true, bz, vlo, r * 8u, PLANE_B + r * 8u);
+    }
+  } else if (FORCE) {
+    row_unpack_emit<N, true>(lds, t, true, bf, vlo, 0u, 0u);
+  }
+
+  if (a.flags & ROW_SPEED) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) smax = fmaxf(smax, __shfl_xor(smax, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = smax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = wmax[0];
+      for (int w = 1; w < ROWFP_GROUPS * T / 64; ++w) m = fmaxf(m, wmax[w]);
+      atomicMax(&I->st.speed_bits[bx % DevState::SPEED_SLOTS], __float_as_uint(m));
+    }
+    if (bad) atomicOr(&I->st.nonfinite, 1u);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1095,6 +1259,21 @@ struct Impl {
 
   // Returns true when the products were written in the sum / difference form (S', D'): the
   // stage-0 kinds, which read the omega', s', d' planes. vel and force apply at stage 0 only.
+  static void launch_rows_fp(Solver* s, const RowArgs& a, bool sd, bool vel, bool force) {
+    if constexpr (RowFp<N>::on) {
+      constexpr int FG = (N / 2 + 7) / 8 * 8;
+      const dim3 grid(FG, s->batch), block(ROWFP_GROUPS * L::T);
+      if (sd && force)
+        hipLaunchKernelGGL((k_rows_fp<N, ROWS_SINGLE_FORCE>), grid, block, 0, s->stream, a);
+      else if (sd && vel)
+        hipLaunchKernelGGL((k_rows_fp<N, ROWS_SINGLE>), grid, block, 0, s->stream, a);
+      else if (sd)
+        hipLaunchKernelGGL((k_rows_fp<N, ROWS_PAIRED_SD>), grid, block, 0, s->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_fp<N, ROWS_PAIRED>), grid, block, 0, s->stream, a);
+    }
+  }
+
   static bool rows(Solver* s, int stage, bool speed, bool vel, bool force) {
     RowArgs a{};
     a.mu = s->mixed_u;
@@ -1115,7 +1294,9 @@ struct Impl {
     const bool sd = stage == 0;
     if (!sd && (speed || vel || force))
       throw std::logic_error("row kernel: speed, dye velocity and pointer forcing are stage-0 work");
-    if (sd && force) {
+    if (RowFp<N>::on && s->rows_fp) {
+      launch_rows_fp(s, a, sd, vel, force);
+    } else if (sd && force) {
       hipLaunchKernelGGL((k_rows<N, G, ROWS_SINGLE_FORCE>), dim3(grid, s->batch), dim3(G * L::T), 0,
                          s->stream, a);
     } else if (sd && vel) {

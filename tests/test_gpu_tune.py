@@ -55,6 +55,34 @@ def test_pointer_forcing_every_width(s2t, n):
         assert np.array_equal(got, ref), (n, w)
 
 
+@pytest.mark.parametrize("n", [512, 1024])
+@pytest.mark.parametrize("case", ["plain", "dye", "pointer"])
+def test_row_kernels_bit_identical(s2t, n, case):
+    """The field-parallel row kernel (three thread groups per row pair) and the one-group row
+    kernel give the same state (and dye) bit for bit: paired stages, stage 0 with the dye velocity,
+    and stage 0 with pointer forcing."""
+    res = []
+    for fp in (True, False):
+        with s2t.Solver(n) as s:
+            s.configure(**PHYS)
+            s.seed_spectrum(k0=max(2.0, n / 32), amplitude=5.0, seed=4)
+            s.set_row_kernel(fp)
+            if case == "pointer":
+                s.set_splats(SPLATS)
+            if case == "dye":
+                s.paint_dye("bands")
+                for _ in range(3):
+                    s.step(1, dye=True)
+                res.append((s.read_state(), s.read_dye()))
+            else:
+                s.step(3)
+                res.append((s.read_state(), None))
+    assert np.abs(res[0][0]).max() > 0
+    assert np.array_equal(res[0][0], res[1][0])
+    if case == "dye":
+        assert np.array_equal(res[0][1].view(np.uint16), res[1][1].view(np.uint16))
+
+
 def test_bad_width_rejected(s2t):
     with s2t.Solver(256) as s:
         with pytest.raises(s2t.S2TError, match="not compiled"):

@@ -121,43 +121,39 @@ def ensemble_params(i, n):
 
 def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
     """64 independent members (32 x 256^2, 24 x 512^2, 8 x 1024^2) advanced together by the
-    batch path (one launch per kernel and size class, hipGraph replay), against a loop over the
-    plain single-instance path on the same GPU."""
+    batch path: grouped launches (the default; six launches per step, each over every member,
+    hipGraph replay), then one launch per kernel and size class on concurrent graph branches
+    (default column-group widths, and the widths the tuner picks per class), against a loop over
+    the plain single-instance path on the same GPU."""
     sizes = ENSEMBLE_SIZES
     pts = sum(n * n for n in sizes)
+
+    def timed(b):
+        b.step(warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        return time.perf_counter() - t0
+
     with s2t.Batch(sizes) as b:
         b.use_graphs(True)
         for i, n in enumerate(sizes):
             b.configure(i, **ensemble_params(i, n))
             b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=1000 + i)
-        b.step(warmup)
-        b.sync()
-        default_widths = b.column_widths()
-        # the default column-group widths, then the widths the tuner picks for each size class
-        # with its member count (tuning runs on scratch solvers, outside the timed regions)
-        t0 = time.perf_counter()
-        b.step(steps)
-        b.sync()
-        el_default = time.perf_counter() - t0
-        if tune:
-            widths = b.autotune(steps=20)
-            b.step(warmup)
-            b.sync()
-            t0 = time.perf_counter()
-            b.step(steps)
-            b.sync()
-            el = time.perf_counter() - t0
-        else:
-            widths, el = default_widths, el_default
-        # grouped launches: six launches per step over every member (s2t_batch_use_grouped)
-        b.use_grouped(True)
-        b.step(warmup)
-        b.sync()
-        t0 = time.perf_counter()
-        b.step(steps)
-        b.sync()
-        el_grouped = time.perf_counter() - t0
+        el = timed(b)
         b.use_grouped(False)
+        default_widths = b.column_widths()
+        el_class = timed(b)
+        per_class = {"value": round(pts * steps / el_class / 1e6, 2),
+                     "ms_per_step": round(el_class * 1e3 / steps, 4),
+                     "widths": {str(k): v for k, v in sorted(default_widths.items())}}
+        if tune:
+            # tuning runs on scratch solvers, outside the timed regions
+            widths = b.autotune(steps=20)
+            el_tuned = timed(b)
+            per_class["tuned"] = {"value": round(pts * steps / el_tuned / 1e6, 2),
+                                  "widths": {str(k): v for k, v in sorted(widths.items())}}
         bad = [i for i in range(len(sizes)) if not np.isfinite(b.diagnostics(i)["energy"])]
     if bad:
         raise RuntimeError("non-finite ensemble members %s" % bad)
@@ -182,12 +178,8 @@ def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
             "data": "synthetic: forced turbulence members, seeds 1000+i",
             "config": {"workload": "batched ensemble, 64 members, 16,777,216 points",
                        "members": len(sizes), "points": pts,
-                       "column_widths": {str(k): v for k, v in sorted(widths.items())}},
-            "default_widths": {"value": round(pts * steps / el_default / 1e6, 2),
-                               "widths": {str(k): v for k, v in sorted(default_widths.items())}},
-            "grouped": {"value": round(pts * steps / el_grouped / 1e6, 2),
-                        "ms_per_step": round(el_grouped * 1e3 / steps, 4),
-                        "launches_per_step": 6},
+                       "launches": "grouped: 6 per step over all members"},
+            "per_class_graph": per_class,
             "plain_loop": {"value": round(plain_rate, 2), "steps_per_member": plain_steps,
                            "seconds": round(el_plain, 3)},
             "speedup_vs_plain_loop": round(batch_rate / plain_rate, 3)}
@@ -654,7 +646,7 @@ def main(argv=None):
         # loop over the plain single-instance path on the same GPU
         ens = bench_ensemble(s2t, 100, 10, 20, tune=False)
         out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "plain_loop",
-                                               "speedup_vs_plain_loop", "grouped")}
+                                               "speedup_vs_plain_loop", "per_class_graph")}
         out["ensemble"]["workload"] = ens["config"]["workload"]
         srv = bench_serve(s2t, [10.0, 30.0, 60.0], 80)
         out["serve"] = {"saturation_mpts_steps_per_s": srv["saturation_mpts_steps_per_s"],

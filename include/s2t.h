@@ -268,10 +268,11 @@ int s2t_batch_diagnostics(s2t_batch* b, int member, s2t_diag* out, double* spect
 /* Capture each RK3 step of all members as a hipGraph (default on) or launch eagerly. */
 int s2t_batch_use_graphs(s2t_batch* b, int enable);
 /* Grouped launches (no counterpart in the reference, whose instances are single solvers): the
- * classes of sizes 256, 512 and 1024 (up to four) are stepped by six launches per step that each
- * cover every instance of those classes, instead of six launches per class on concurrent graph
- * branches; other sizes keep their own launches. Results are bit-identical either way. Default
- * off. */
+ * classes of sizes 256, 512 and 1024 are stepped by six launches per step that each cover every
+ * instance of those classes, instead of six launches per class on concurrent graph branches;
+ * other sizes keep their own launches. Results are bit-identical either way. Default on; the
+ * column-group widths of s2t_batch_autotune / s2t_batch_column_width apply to the classes on
+ * their own launches. */
 int s2t_batch_use_grouped(s2t_batch* b, int enable);
 /* nsteps steps of every member; enqueues only. */
 int s2t_batch_step(s2t_batch* b, int nsteps);

@@ -6,9 +6,11 @@
 // class branches fork from, and join back to, the batch's main stream, so classes run
 // concurrently and a replay needs no host work. dt, stepIndex and the forcing phases live in the
 // per-member device scalars, so a replayed graph needs no parameter updates.
-// Grouped mode: the classes of sizes 256, 512 and 1024 are stepped together by grouped launches
-// (one launch per row or column pass covering every instance of those classes, on the main
-// stream), the other classes on their branches as before.
+// Grouped mode (the default): the classes of sizes 256, 512 and 1024 are stepped together by
+// grouped launches (one launch per row or column pass covering every instance of those classes,
+// on the main stream), the other classes on their branches as before. On the 64-member mix of
+// 256^2, 512^2 and 1024^2 members it runs 32,100-32,200 Mpts-steps/s against 31,100-31,200 for
+// the per-class branches (400-step runs, alternated, on three boxes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +33,7 @@ struct Batch {
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
   bool use_graphs = true;
-  bool grouped = false;
+  bool grouped = true;
   bool captured = false;
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};

@@ -27,11 +27,13 @@ def run_plain(s2t, n, params, seed, steps, k0=None):
         return s.read_state(), s.diagnostics()
 
 
+@pytest.mark.parametrize("grouped", [True, False])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_batch_members_match_plain(s2t, graphs):
+def test_batch_members_match_plain(s2t, graphs, grouped):
     steps = 12
     with s2t.Batch(SIZES) as b:
         b.use_graphs(graphs)
+        b.use_grouped(grouped)
         for i, n in enumerate(SIZES):
             b.configure(i, **member_params(i))
             b.seed_spectrum(i, k0=n / 10, amplitude=6.0, seed=1000 + i)

@@ -41,13 +41,20 @@ def measured_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", "step_kernels_%d_forced_turbulence.md" % n)
     if not os.path.exists(path):
         return None
-    prefix = "s2t::%s<%d," % (kernel, n)
-    suffix = ",0,false>" if kernel == "k_cols" else ",2>"
+    # the kernel's one-group or field-parallel form (k_cols<n,C,0,false>, k_cols_fp<n,G,false>;
+    # k_rows<n,G,2>, k_rows_fp<n,2>)
+    prefixes = ("s2t::%s<%d," % (kernel, n), "s2t::%s_fp<%d," % (kernel, n))
+    suffixes = (",0,false>",) if kernel == "k_cols" else (",2>",)
     with open(path) as f:
         for line in f:
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1].startswith(prefix) \
-                    and parts[1].endswith(suffix):
+            if len(parts) != 3 or parts[0] != "TRAFFIC":
+                continue
+            name = parts[1]
+            if name.startswith(prefixes[0]) and name.endswith(suffixes[0]):
+                return float(parts[2])
+            if name.startswith(prefixes[1]) and (kernel == "k_rows" and name.endswith(",2>")
+                                                 or kernel == "k_cols" and name.endswith(",false>")):
                 return float(parts[2])
     return None
 

@@ -224,6 +224,39 @@ def test_js_batch_members_equal_plain_solvers():
     assert str(out["tuned"]["width"]) in out["tuned"]["ms"]
 
 
+RESET_JS = r"""
+const m = require('./index.js');
+(async () => {
+  const device = await m.requestDevice();
+  const s = await m.Solver.create(device, { n: 64, dyeScale: 2 });
+  s.reset({ k0: 8, amplitude: 5, seed: 3 });              // bands painted
+  s.step(3, { dye: true });
+  const before = s.currentDye;
+  let painted = 0;
+  for (let k = 0; k < before.length; k++) if (before[k] !== 0) painted++;
+  s.reset({ k0: 8, amplitude: 5, seed: 3, dyePattern: 'none' });
+  const after = s.currentDye;
+  let nonzero = 0;
+  for (let k = 0; k < after.length; k++) if (after[k] !== 0) nonzero++;
+  console.log(JSON.stringify({ painted, nonzero, len: after.length, parity: s.dyeParity,
+                               primed: s.primed, step: s.stepIndex }));
+  s.destroy();
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+@pytest.mark.gpu
+def test_js_reset_without_dye_pattern_blanks_the_dye():
+    """reset({dyePattern: 'none'}) writes a blank dye and restarts the dye parity, as
+    solver.js:420-436 does, instead of keeping the previous run's dye."""
+    import json
+    r = subprocess.run([NODE, "-e", RESET_JS], cwd=JS, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["painted"] > 0 and out["len"] == 128 * 128 * 4
+    assert out["nonzero"] == 0 and out["parity"] == 0 and not out["primed"] and out["step"] == 0
+
+
 FAIL_JS = r"""
 const m = require('./index.js');
 const phys = { beta: 5, drag: 0.01, viscosity: 1, hyperP: 4, cfl: 0.5, dtMin: 1e-5, dtMax: 0.02 };

@@ -436,17 +436,33 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
   return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
                      a.w + (b.w - a.w) * f);
 }
-// Bilinear sample at unit position (px, py) of a texel source `at(ix, iy)` (float4; repeat
-// addressing is the source's), texel centres at (i + 0.5)/m.
-template <class At>
-__device__ __forceinline__ float4 sample_at(At&& at, int m, float px, float py) {
+// the same with alpha 1 (the dye's alpha is set to 1 after every step, so the fused kernel
+// carries only colour through its intermediate values)
+__device__ __forceinline__ float4 lerp3(float4 a, float4 b, float f) {
+  return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f, 1.f);
+}
+// Bilinear sample at unit position (px, py) from a corner source `at4(ix, iy, c)` that fills
+// c[0..3] with the texels (ix, iy), (ix+1, iy), (ix, iy+1), (ix+1, iy+1) (float4; repeat
+// addressing is the source's), texel centres at (i + 0.5)/m. RGB: alpha is 1, not interpolated.
+template <bool RGB = false, class At4>
+__device__ __forceinline__ float4 sample4_at(At4&& at4, int m, float px, float py) {
   const float gx = px * (float)m - 0.5f, gy = py * (float)m - 0.5f;
   const float bx = floorf(gx), by = floorf(gy);
   const float fx = gx - bx, fy = gy - by;
-  const int ix = (int)bx, iy = (int)by;
-  const float4 top = lerp4(at(ix, iy), at(ix + 1, iy), fx);
-  const float4 bot = lerp4(at(ix, iy + 1), at(ix + 1, iy + 1), fx);
-  return lerp4(top, bot, fy);
+  float4 c[4];
+  at4((int)bx, (int)by, c);
+  if constexpr (RGB) return lerp3(lerp3(c[0], c[1], fx), lerp3(c[2], c[3], fx), fy);
+  return lerp4(lerp4(c[0], c[1], fx), lerp4(c[2], c[3], fx), fy);
+}
+// the same from a texel source `at(ix, iy)`
+template <bool RGB = false, class At>
+__device__ __forceinline__ float4 sample_at(At&& at, int m, float px, float py) {
+  return sample4_at<RGB>([&](int ix, int iy, float4(&c)[4]) {
+    c[0] = at(ix, iy);
+    c[1] = at(ix + 1, iy);
+    c[2] = at(ix, iy + 1);
+    c[3] = at(ix + 1, iy + 1);
+  }, m, px, py);
 }
 __device__ __forceinline__ float4 sample_tex(const ushort4* __restrict__ tex, int m, float px,
                                              float py) {
@@ -456,28 +472,38 @@ __device__ __forceinline__ float2 vel_at(const float2* __restrict__ vel, int n, 
   const int x = ix & (n - 1), y = iy & (n - 1);
   return vel[(size_t)y * n + x];
 }
-// bilinear velocity at unit position (px, py) from a cell source `va(ix, iy)` (cell centres at
-// (i + 0.5)/n, repeat addressing is the source's)
-template <class Va>
-__device__ __forceinline__ float2 sample_vel_at(Va&& va, int n, float px, float py) {
+// bilinear velocity at unit position (px, py) from a corner source `va4(ix, iy, c)` (cells
+// (ix, iy), (ix+1, iy), (ix, iy+1), (ix+1, iy+1) into c[0..3]; cell centres at (i + 0.5)/n,
+// repeat addressing is the source's)
+template <class Va4>
+__device__ __forceinline__ float2 sample_vel4_at(Va4&& va4, int n, float px, float py) {
   const float gx = px * (float)n - 0.5f, gy = py * (float)n - 0.5f;
   const float bx = floorf(gx), by = floorf(gy);
   const float fx = gx - bx, fy = gy - by;
-  const int ix = (int)bx, iy = (int)by;
-  const float2 v00 = va(ix, iy), v10 = va(ix + 1, iy);
-  const float2 v01 = va(ix, iy + 1), v11 = va(ix + 1, iy + 1);
-  const float2 a = make_float2(v00.x + (v10.x - v00.x) * fx, v00.y + (v10.y - v00.y) * fx);
-  const float2 b = make_float2(v01.x + (v11.x - v01.x) * fx, v01.y + (v11.y - v01.y) * fx);
+  float2 v[4];
+  va4((int)bx, (int)by, v);
+  const float2 a = make_float2(v[0].x + (v[1].x - v[0].x) * fx, v[0].y + (v[1].y - v[0].y) * fx);
+  const float2 b = make_float2(v[2].x + (v[3].x - v[2].x) * fx, v[2].y + (v[3].y - v[2].y) * fx);
   return make_float2(a.x + (b.x - a.x) * fy, a.y + (b.y - a.y) * fy);
 }
 __device__ __forceinline__ float fract1(float x) { return x - floorf(x); }
-// RK2 midpoint back-trace (dye.wgsl:40-45)
+// RK2 midpoint back-trace (dye.wgsl:40-45) through a corner source
+template <class Va4>
+__device__ __forceinline__ float2 trace4_at(Va4&& va4, int n, float px, float py, float h) {
+  const float2 v1 = sample_vel4_at(va4, n, px, py);
+  const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
+  const float2 v2 = sample_vel4_at(va4, n, mx, my);
+  return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+}
+// the same through a cell source `va(ix, iy)`
 template <class Va>
 __device__ __forceinline__ float2 trace_at(Va&& va, int n, float px, float py, float h) {
-  const float2 v1 = sample_vel_at(va, n, px, py);
-  const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
-  const float2 v2 = sample_vel_at(va, n, mx, my);
-  return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+  return trace4_at([&](int ix, int iy, float2(&c)[4]) {
+    c[0] = va(ix, iy);
+    c[1] = va(ix + 1, iy);
+    c[2] = va(ix, iy + 1);
+    c[3] = va(ix + 1, iy + 1);
+  }, n, px, py, h);
 }
 __device__ __forceinline__ float2 trace(const float2* __restrict__ vel, int n, float px, float py,
                                         float h) {
@@ -511,7 +537,9 @@ __global__ void k_dye_advect(const ushort4* __restrict__ src, ushort4* __restric
   const int x = xy.x, y = xy.y;
   if (x >= m || y >= m) return;
   const float h = sign * (float)dye_dt(dev, dt) / TWO_PI_F;
-  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+  // m is a power of two: the product by 1/m is the quotient, exactly
+  const float im = 1.f / (float)m;
+  const float px = ((float)x + 0.5f) * im, py = ((float)y + 0.5f) * im;
   const float2 s = trace(vel, n, px, py, h);
   dst[(size_t)y * m + x] = to_half4(sample_tex(src, m, s.x, s.y));
 }
@@ -557,7 +585,9 @@ __global__ void k_dye_corr(const ushort4* __restrict__ src, const ushort4* __res
   const int x = xy.x, y = xy.y;
   if (x >= m || y >= m) return;
   const float h = (float)dye_dt(dev, dt) / TWO_PI_F;
-  const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+  // m is a power of two: the product by 1/m is the quotient, exactly
+  const float im = 1.f / (float)m;
+  const float px = ((float)x + 0.5f) * im, py = ((float)y + 0.5f) * im;
   const float2 s = trace(vel, n, px, py, h);
   const float4 a = sample_tex(src, m, s.x, s.y);
   const float4 b = sample_tex(aux, m, s.x, s.y);
@@ -597,14 +627,23 @@ struct DyeVel {
   static constexpr int CELLS = (DYE_T + R - 1) / R + 2 * LO + 1;
 };
 
-__device__ __forceinline__ float4 half4(ushort4 v) {
+// colour of a binary16 texel, alpha 1
+__device__ __forceinline__ float4 half3(ushort4 v) {
   return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
-                     __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
+                     __half2float(__ushort_as_half(v.z)), 1.f);
+}
+// binary16 of a colour, alpha 1
+__device__ __forceinline__ ushort4 to_half3(float4 v) {
+  return make_ushort4(__half_as_ushort(__float2half(v.x)), __half_as_ushort(__float2half(v.y)),
+                      __half_as_ushort(__float2half(v.z)), 0x3c00);
 }
 
 // The stages of one tile, after phi0 and the velocity cells are staged. CHECKED: every lookup
 // checks its staged region and computes a sample outside it on the spot; otherwise (the tile's
-// feet are known to stay within one texel) lookups go straight to LDS.
+// feet are known to stay within one texel) lookups go straight to LDS, the four corners of a
+// stencil from one address. Only colour is carried: every output has alpha 1 whatever the alpha
+// of the intermediate values, so they are given alpha 1 (the colour is the three-pass colour bit
+// for bit).
 template <int R, bool CHECKED>
 __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
                                          const float2* __restrict__ vel, const ushort4* s0,
@@ -615,50 +654,117 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
   using V = DyeVel<R>;
   const int tid = (int)threadIdx.x;
   const int mask = m - 1, nmask = n - 1;
+  // A staged region holds its points at relative index (i - origin) & mask in [0, size) (a grid
+  // smaller than the region is staged periodically), so with the stencil's first corner inside,
+  // its neighbours are the next index and the next row.
   auto va = [=](int ix, int iy) -> float2 {
     const int rx = (ix - vx0) & nmask, ry = (iy - vy0) & nmask;
     if (!CHECKED || (rx < V::CELLS && ry < V::CELLS)) return sv[ry * V::CELLS + rx];
     return vel_at(vel, n, ix, iy);
   };
+  auto va4 = [=](int ix, int iy, float2(&c)[4]) {
+    if constexpr (!CHECKED) {
+      const float2* q = sv + ((iy - vy0) & nmask) * V::CELLS + ((ix - vx0) & nmask);
+      c[0] = q[0];
+      c[1] = q[1];
+      c[2] = q[V::CELLS];
+      c[3] = q[V::CELLS + 1];
+    } else {
+      c[0] = va(ix, iy);
+      c[1] = va(ix + 1, iy);
+      c[2] = va(ix, iy + 1);
+      c[3] = va(ix + 1, iy + 1);
+    }
+  };
   auto phi0 = [=](int ix, int iy) -> float4 {
     const int rx = (ix - x0 + DYE_E0) & mask, ry = (iy - y0 + DYE_E0) & mask;
-    if (!CHECKED || (rx < DYE_R0 && ry < DYE_R0)) return half4(s0[ry * DYE_R0 + rx]);
-    return texel(src, m, ix, iy);
+    if (!CHECKED || (rx < DYE_R0 && ry < DYE_R0)) return half3(s0[ry * DYE_R0 + rx]);
+    return half3(src[(size_t)(iy & mask) * m + (ix & mask)]);
   };
-  auto centre = [=](int i) { return ((float)(i & mask) + 0.5f) / (float)m; };
+  auto phi0_4 = [=](int ix, int iy, float4(&c)[4]) {
+    if constexpr (!CHECKED) {
+      const ushort4* q = s0 + ((iy - y0 + DYE_E0) & mask) * DYE_R0 + ((ix - x0 + DYE_E0) & mask);
+      c[0] = half3(q[0]);
+      c[1] = half3(q[1]);
+      c[2] = half3(q[DYE_R0]);
+      c[3] = half3(q[DYE_R0 + 1]);
+    } else {
+      c[0] = phi0(ix, iy);
+      c[1] = phi0(ix + 1, iy);
+      c[2] = phi0(ix, iy + 1);
+      c[3] = phi0(ix + 1, iy + 1);
+    }
+  };
+  // texel centre; m is a power of two, so the product by 1/m is the quotient, exactly
+  const float im = 1.f / (float)m;
+  auto centre = [=](int i) { return ((float)(i & mask) + 0.5f) * im; };
   // phi1 of texel (ix, iy): the forward advection of phi0, rounded to binary16
   auto phi1_of = [=](int ix, int iy) -> ushort4 {
-    const float2 f = trace_at(va, n, centre(ix), centre(iy), hf);
-    return to_half4(sample_at(phi0, m, f.x, f.y));
+    const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hf);
+    return to_half3(sample4_at<true>(phi0_4, m, f.x, f.y));
   };
   // the same at a point of the phi1 region, keeping the forward foot of the tile's own texels
   // (the trace of the correction stage)
   auto phi1_keep = [=](int i) -> ushort4 {
     const int rx = i % DYE_R1, ry = i / DYE_R1;
     const int ix = x0 - DYE_E1 + rx, iy = y0 - DYE_E1 + ry;
-    const float2 f = trace_at(va, n, centre(ix), centre(iy), hf);
+    const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hf);
     const int tx = rx - DYE_E1, ty = ry - DYE_E1;
     if (tx >= 0 && tx < DYE_T && ty >= 0 && ty < DYE_T) sfoot[ty * DYE_T + tx] = f;
-    return to_half4(sample_at(phi0, m, f.x, f.y));
+    return to_half3(sample4_at<true>(phi0_4, m, f.x, f.y));
   };
   auto phi1 = [=](int ix, int iy) -> float4 {
     const int rx = (ix - x0 + DYE_E1) & mask, ry = (iy - y0 + DYE_E1) & mask;
-    if (!CHECKED || (rx < DYE_R1 && ry < DYE_R1)) return half4(s1[ry * DYE_R1 + rx]);
-    return half4(phi1_of(ix, iy));
+    if (!CHECKED || (rx < DYE_R1 && ry < DYE_R1)) return half3(s1[ry * DYE_R1 + rx]);
+    return half3(phi1_of(ix, iy));
+  };
+  auto phi1_4 = [=](int ix, int iy, float4(&c)[4]) {
+    if constexpr (!CHECKED) {
+      const ushort4* q = s1 + ((iy - y0 + DYE_E1) & mask) * DYE_R1 + ((ix - x0 + DYE_E1) & mask);
+      c[0] = half3(q[0]);
+      c[1] = half3(q[1]);
+      c[2] = half3(q[DYE_R1]);
+      c[3] = half3(q[DYE_R1 + 1]);
+    } else {
+      c[0] = phi1(ix, iy);
+      c[1] = phi1(ix + 1, iy);
+      c[2] = phi1(ix, iy + 1);
+      c[3] = phi1(ix + 1, iy + 1);
+    }
   };
   // phi1 without the staged values (once phi_hat has taken their space)
-  auto phi1_fresh = [=](int ix, int iy) -> float4 { return half4(phi1_of(ix, iy)); };
-  // phi_hat of texel (ix, iy): the backward advection of phi1 (from the accessor p1), rounded to
-  // binary16
-  auto phih_from = [=](auto p1, int ix, int iy) -> ushort4 {
-    const float2 f = trace_at(va, n, centre(ix), centre(iy), hb);
-    return to_half4(sample_at(p1, m, f.x, f.y));
+  auto phi1_fresh = [=](int ix, int iy) -> float4 { return half3(phi1_of(ix, iy)); };
+  auto phi1_fresh4 = [=](int ix, int iy, float4(&c)[4]) {
+    c[0] = phi1_fresh(ix, iy);
+    c[1] = phi1_fresh(ix + 1, iy);
+    c[2] = phi1_fresh(ix, iy + 1);
+    c[3] = phi1_fresh(ix + 1, iy + 1);
   };
-  auto phih_of = [=](int ix, int iy) -> ushort4 { return phih_from(phi1, ix, iy); };
+  // phi_hat of texel (ix, iy): the backward advection of phi1 (from the corner source p1),
+  // rounded to binary16
+  auto phih_from = [=](auto p1, int ix, int iy) -> ushort4 {
+    const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hb);
+    return to_half3(sample4_at<true>(p1, m, f.x, f.y));
+  };
+  auto phih_of = [=](int ix, int iy) -> ushort4 { return phih_from(phi1_4, ix, iy); };
   auto phih = [=](int ix, int iy) -> float4 {
     const int rx = (ix - x0 + DYE_E2) & mask, ry = (iy - y0 + DYE_E2) & mask;
-    if (!CHECKED || (rx < DYE_R2 && ry < DYE_R2)) return half4(s2[ry * DYE_R2 + rx]);
-    return half4(phih_from(phi1_fresh, ix, iy));
+    if (!CHECKED || (rx < DYE_R2 && ry < DYE_R2)) return half3(s2[ry * DYE_R2 + rx]);
+    return half3(phih_from(phi1_fresh4, ix, iy));
+  };
+  auto phih4 = [=](int ix, int iy, float4(&c)[4]) {
+    if constexpr (!CHECKED) {
+      const ushort4* q = s2 + ((iy - y0 + DYE_E2) & mask) * DYE_R2 + ((ix - x0 + DYE_E2) & mask);
+      c[0] = half3(q[0]);
+      c[1] = half3(q[1]);
+      c[2] = half3(q[DYE_R2]);
+      c[3] = half3(q[DYE_R2 + 1]);
+    } else {
+      c[0] = phih(ix, iy);
+      c[1] = phih(ix + 1, iy);
+      c[2] = phih(ix, iy + 1);
+      c[3] = phih(ix + 1, iy + 1);
+    }
   };
   // two items per thread per iteration: independent chains the scheduler can interleave
   for (int i = tid; i < DYE_R1 * DYE_R1; i += 2 * DYE_THREADS) {
@@ -687,17 +793,21 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
   auto out = [&](int i) {
     const int x = x0 + i % DYE_T, y = y0 + i / DYE_T;
     if (x >= m || y >= m) return;
-    const float px = ((float)x + 0.5f) / (float)m, py = ((float)y + 0.5f) / (float)m;
+    const float px = ((float)x + 0.5f) * im, py = ((float)y + 0.5f) * im;
     const float2 s = sfoot[i];  // trace_at(va, n, px, py, hf), from the phi1 stage
-    const float4 a = sample_at(phi0, m, s.x, s.y);
-    const float4 bb = sample_at(phih, m, s.x, s.y);
+    // the phi0 stencil at the foot: the advected value and the limiter's bounds
+    float4 c[4];
+    const float4 a = sample4_at<true>([&](int ix, int iy, float4(&cc)[4]) {
+      phi0_4(ix, iy, cc);
+      c[0] = cc[0];
+      c[1] = cc[1];
+      c[2] = cc[2];
+      c[3] = cc[3];
+    }, m, s.x, s.y);
+    const float4 bb = sample4_at<true>(phih4, m, s.x, s.y);
     const float4 corr = make_float4(1.5f * a.x - 0.5f * bb.x, 1.5f * a.y - 0.5f * bb.y,
-                                    1.5f * a.z - 0.5f * bb.z, 1.5f * a.w - 0.5f * bb.w);
-    const float gx = s.x * (float)m - 0.5f, gy = s.y * (float)m - 0.5f;
-    const int ix = (int)floorf(gx), iy = (int)floorf(gy);
-    const float4 c00 = phi0(ix, iy), c10 = phi0(ix + 1, iy);
-    const float4 c01 = phi0(ix, iy + 1), c11 = phi0(ix + 1, iy + 1);
-    dst[(size_t)y * m + x] = to_half4(dye_correct(corr, c00, c10, c01, c11, fade, sp, px, py));
+                                    1.5f * a.z - 0.5f * bb.z, 1.f);
+    dst[(size_t)y * m + x] = to_half4(dye_correct(corr, c[0], c[1], c[2], c[3], fade, sp, px, py));
   };
   for (int i = tid; i < DYE_T * DYE_T; i += 2 * DYE_THREADS) {
     out(i);

@@ -114,6 +114,7 @@ struct Chunk {
   double ms_per_step = 1.0;       // running estimate
   hipEvent_t e0 = nullptr, e1 = nullptr;
   size_t bytes = 0;
+  Inst* host_inst = nullptr;      // pinned copy of the slots' parameters and clocks, per poll
 };
 
 struct Server {
@@ -186,6 +187,8 @@ static Chunk* new_chunk(int n, int slots) {
     c->bytes = chunk_bytes(n, slots);
     hipchk(hipEventCreate(&c->e0), "hipEventCreate");
     hipchk(hipEventCreate(&c->e1), "hipEventCreate");
+    hipchk(hipHostMalloc((void**)&c->host_inst, sizeof(Inst) * slots, hipHostMallocDefault),
+           "hipHostMalloc");
     // a new chunk starts with every slot idle
     for (int k = 0; k < slots; ++k) {
       const int zero = 0;
@@ -195,6 +198,7 @@ static Chunk* new_chunk(int n, int slots) {
   } catch (...) {
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->host_inst) (void)hipHostFree(c->host_inst);
     solver_free(c->s);
     delete c;
     throw;
@@ -206,7 +210,8 @@ static void free_chunk(Chunk* c) {
   if (!c) return;
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
-  solver_free(c->s);
+  solver_free(c->s);  // waits for the chunk's stream, so no copy into host_inst is in flight
+  if (c->host_inst) (void)hipHostFree(c->host_inst);
   delete c;
 }
 
@@ -413,6 +418,10 @@ static void worker_loop(Server* sv) {
           hipchk(hipEventRecord(c->e0, c->s->stream), "hipEventRecord");
           for (int i = 0; i < k; ++i) step_once(c->s, false);
           hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
+          // every slot's clock in one copy, behind the steps
+          hipchk(hipMemcpyAsync(c->host_inst, c->s->inst, sizeof(Inst) * c->s->batch,
+                                hipMemcpyDeviceToHost, c->s->stream),
+                 "hipMemcpyAsync");
           ran.push_back({c, k});
         }
       // 3. poll: per-slot device flags tell which requests retired
@@ -424,7 +433,7 @@ static void worker_loop(Server* sv) {
         c.ms_per_step = 0.5 * c.ms_per_step + 0.5 * std::max(1e-3, (double)ms / pr.second);
         for (int slot = 0; slot < c.s->batch; ++slot) {
           if (c.slot_req[slot] < 0) continue;
-          const DevState d = read_clock_slot(c.s, slot);
+          const DevState& d = c.host_inst[slot].st;
           if (!d.active) {
             const uint64_t id = (uint64_t)c.slot_req[slot];
             complete(sv, c, slot);

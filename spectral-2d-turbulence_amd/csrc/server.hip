@@ -110,6 +110,7 @@ struct Chunk {
   Solver* s = nullptr;
   std::vector<int64_t> slot_req;  // request id per slot, -1 when free
   std::vector<int> slot_phase;    // chunk buffer phase when the slot's request was admitted
+  std::vector<char> slot_fresh;   // admitted after this poll's clock copy was enqueued
   int active = 0;
   double ms_per_step = 1.0;       // running estimate
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -184,6 +185,7 @@ static Chunk* new_chunk(int n, int slots) {
     c->s = solver_new(n, 1, slots, false);
     c->slot_req.assign(slots, -1);
     c->slot_phase.assign(slots, 0);
+    c->slot_fresh.assign(slots, 0);
     c->bytes = chunk_bytes(n, slots);
     hipchk(hipEventCreate(&c->e0), "hipEventCreate");
     hipchk(hipEventCreate(&c->e1), "hipEventCreate");
@@ -330,8 +332,7 @@ static void cancel_slot(Server* sv, Chunk& c, int slot) {
 static void worker_loop(Server* sv) {
   try {
     while (true) {
-      // 1. admission, FIFO: each pending request takes a slot wherever `place` finds one; one that
-      // fits nowhere yet keeps its place in the queue
+      // 1. the queue, or a wait for work
       std::vector<Request> queue;
       std::vector<uint64_t> dropped;  // queued requests cancelled by the shutdown
       {
@@ -366,6 +367,42 @@ static void worker_loop(Server* sv) {
         queue.assign(sv->pending.begin(), sv->pending.end());
       }
       for (uint64_t id : dropped) notify(sv, id, S2T_REQ_CANCELLED);
+      // cancellations of running requests, between steps (every chunk stream is idle here)
+      std::vector<uint64_t> stopped;
+      for (auto& kv : sv->cls)
+        for (Chunk* c : kv.second)
+          for (int slot = 0; slot < (int)c->slot_req.size(); ++slot) {
+            if (c->slot_req[slot] < 0) continue;
+            bool want;
+            {
+              std::lock_guard<std::mutex> lk(sv->mu);
+              want = sv->rec[(uint64_t)c->slot_req[slot]].cancel;
+            }
+            if (want) {
+              stopped.push_back((uint64_t)c->slot_req[slot]);
+              cancel_slot(sv, *c, slot);
+            }
+          }
+      for (uint64_t id : stopped) notify(sv, id, S2T_REQ_CANCELLED);
+      // 2. step every chunk with active slots, each on its own stream (enqueued only)
+      std::vector<std::pair<Chunk*, int>> ran;
+      for (auto& kv : sv->cls)
+        for (Chunk* c : kv.second) {
+          if (c->active == 0) continue;
+          const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c->ms_per_step)));
+          ensure_primed(c->s);
+          hipchk(hipEventRecord(c->e0, c->s->stream), "hipEventRecord");
+          for (int i = 0; i < k; ++i) step_once(c->s, false);
+          hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
+          // every slot's clock in one copy, behind the steps
+          hipchk(hipMemcpyAsync(c->host_inst, c->s->inst, sizeof(Inst) * c->s->batch,
+                                hipMemcpyDeviceToHost, c->s->stream),
+                 "hipMemcpyAsync");
+          ran.push_back({c, k});
+        }
+      // 3. admission, FIFO, while the steps just enqueued run: each pending request takes a slot
+      // wherever `place` finds one (its seeding is ordered behind the chunk's steps on the
+      // chunk's stream); one that fits nowhere yet keeps its place in the queue
       std::vector<uint64_t> taken;
       for (const Request& r : queue) {
         {
@@ -389,42 +426,10 @@ static void worker_loop(Server* sv) {
             }
         }
         admit(sv, *c, slot, r);
+        c->slot_fresh[slot] = 1;
         sv->admitted += 1;
       }
-      // cancellations of running requests, between steps (every chunk stream is idle here)
-      std::vector<uint64_t> stopped;
-      for (auto& kv : sv->cls)
-        for (Chunk* c : kv.second)
-          for (int slot = 0; slot < (int)c->slot_req.size(); ++slot) {
-            if (c->slot_req[slot] < 0) continue;
-            bool want;
-            {
-              std::lock_guard<std::mutex> lk(sv->mu);
-              want = sv->rec[(uint64_t)c->slot_req[slot]].cancel;
-            }
-            if (want) {
-              stopped.push_back((uint64_t)c->slot_req[slot]);
-              cancel_slot(sv, *c, slot);
-            }
-          }
-      for (uint64_t id : stopped) notify(sv, id, S2T_REQ_CANCELLED);
-      // 2. step every chunk with active slots, each on its own stream
-      std::vector<std::pair<Chunk*, int>> ran;
-      for (auto& kv : sv->cls)
-        for (Chunk* c : kv.second) {
-          if (c->active == 0) continue;
-          const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c->ms_per_step)));
-          ensure_primed(c->s);
-          hipchk(hipEventRecord(c->e0, c->s->stream), "hipEventRecord");
-          for (int i = 0; i < k; ++i) step_once(c->s, false);
-          hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
-          // every slot's clock in one copy, behind the steps
-          hipchk(hipMemcpyAsync(c->host_inst, c->s->inst, sizeof(Inst) * c->s->batch,
-                                hipMemcpyDeviceToHost, c->s->stream),
-                 "hipMemcpyAsync");
-          ran.push_back({c, k});
-        }
-      // 3. poll: per-slot device flags tell which requests retired
+      // 4. poll: per-slot device flags tell which requests retired
       for (auto& pr : ran) {
         Chunk& c = *pr.first;
         sync(c.s);
@@ -432,7 +437,7 @@ static void worker_loop(Server* sv) {
         hipchk(hipEventElapsedTime(&ms, c.e0, c.e1), "hipEventElapsedTime");
         c.ms_per_step = 0.5 * c.ms_per_step + 0.5 * std::max(1e-3, (double)ms / pr.second);
         for (int slot = 0; slot < c.s->batch; ++slot) {
-          if (c.slot_req[slot] < 0) continue;
+          if (c.slot_req[slot] < 0 || c.slot_fresh[slot]) continue;  // copied before its admission
           const DevState& d = c.host_inst[slot].st;
           if (!d.active) {
             const uint64_t id = (uint64_t)c.slot_req[slot];
@@ -441,6 +446,9 @@ static void worker_loop(Server* sv) {
           }
         }
       }
+      // the next poll's copies are enqueued after this iteration's admissions
+      for (auto& kv : sv->cls)
+        for (Chunk* c : kv.second) std::fill(c->slot_fresh.begin(), c->slot_fresh.end(), 0);
     }
   } catch (const std::exception& e) {
     // the worker stops: every request still queued, being admitted or running fails, and its

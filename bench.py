@@ -90,25 +90,40 @@ def workload_params(n):
                 cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
 
 
-def cpu_baseline(n, params, steps=1):
-    """Time the FP64 NumPy oracle step (same operation order as reference/spectral2d.py)."""
+def cpu_baseline(n, params, steps=1, threads=0):
+    """Time the FP64 NumPy oracle step (same operation order as reference/spectral2d.py) on one
+    thread (numpy.fft), and, with threads > 1, the same steps again with scipy.fft on that many
+    workers and the pointwise stages over row bands (reported under "threaded")."""
     sys.path.insert(0, ROOT)
     from oracle import gpusem
+    from oracle import spectral as sp
     from oracle.spectral import Scheme, nu_from_rate
-    o = Scheme(n, beta=params["beta"], mu=params["drag"],
-               nu=nu_from_rate(params["viscosity"], n, int(params["hyper_p"])),
-               p=int(params["hyper_p"]), cfl=params["cfl"], dt_max=params["dt_max"])
-    o.w = gpusem.fill_spectrum(n, k0=64.0, amplitude=1.0, seed=1).astype(np.complex128)
-    t0 = time.perf_counter()
-    for k in range(steps):
-        dt = o.cfl_dt()
-        f = gpusem.stochastic_forcing(n, params["force_amp"], params["force_k"],
-                                      params["force_width"], k, dt)
-        o.step(dt=dt, forcing=f)
-    el = time.perf_counter() - t0
-    return dict(value=n * n * steps / el / 1e6, unit="Mpts·steps/s", cores=1, kind="port",
-                sample="%d IF-RK3 step(s) at %d^2 of the FP64 NumPy oracle (numpy.fft, one thread),"
-                       " CFL dt and stochastic forcing included, %.1f s" % (steps, n, el))
+
+    def run(workers):
+        sp.set_fft_workers(workers)
+        o = Scheme(n, beta=params["beta"], mu=params["drag"],
+                   nu=nu_from_rate(params["viscosity"], n, int(params["hyper_p"])),
+                   p=int(params["hyper_p"]), cfl=params["cfl"], dt_max=params["dt_max"])
+        o.w = gpusem.fill_spectrum(n, k0=64.0, amplitude=1.0, seed=1).astype(np.complex128)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            dt = o.cfl_dt()
+            f = gpusem.stochastic_forcing(n, params["force_amp"], params["force_k"],
+                                          params["force_width"], k, dt)
+            o.step(dt=dt, forcing=f)
+        return time.perf_counter() - t0
+
+    el = run(1)
+    out = dict(value=n * n * steps / el / 1e6, unit="Mpts·steps/s", cores=1, kind="port",
+               sample="%d IF-RK3 step(s) at %d^2 of the FP64 NumPy oracle (numpy.fft, one thread),"
+                      " CFL dt and stochastic forcing included, %.1f s" % (steps, n, el))
+    if threads > 1:
+        el_t = run(threads)
+        out["threaded"] = dict(value=n * n * steps / el_t / 1e6, cores=threads,
+                               sample="the same steps with scipy.fft on %d workers and the pointwise"
+                                      " stages over %d row bands, %.1f s" % (threads, threads, el_t))
+    sp.set_fft_workers(1)
+    return out
 
 
 ENSEMBLE_SIZES = [256] * 32 + [512] * 24 + [1024] * 8  # 16,777,216 points = 4096^2
@@ -665,7 +680,9 @@ def main(argv=None):
                         "workload": "80 requests, N in {128..2048}^2, steps log-uniform "
                                     "100..5000, four scenarios, 200 GB budget"}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, params, steps=args.cpu_steps)
+        # the host's cores for the threaded variant (at most 16, the box's CPU share)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(n, params, steps=args.cpu_steps, threads=threads)
     print(json.dumps(out))
     return 0
 

@@ -78,22 +78,24 @@ def test_grouped_members_match_plain(s2t, graphs):
         assert diags[i]["stepIndex"] == 8 and diags[i]["time"] == d["time"]
 
 
-def test_batch_reseed_member_between_replays(s2t):
-    """Replacing one member's state re-primes it; the others continue undisturbed."""
-    sizes = [128, 128, 128]
+@pytest.mark.parametrize("n", [128, 256])
+def test_batch_reseed_member_between_replays(s2t, n):
+    """Replacing one member's state re-primes it; the others continue undisturbed (128: a class on
+    its own launches; 256: a class of the grouped launches)."""
+    sizes = [n, n, n]
     with s2t.Batch(sizes) as b:
         for i in range(3):
             b.configure(i, **member_params(i))
             b.seed_spectrum(i, k0=12, amplitude=6.0, seed=7 + i)
         b.step(4)
-        field = random_field(128, k0=10, seed=3, amplitude=4.0)
+        field = random_field(n, k0=10, seed=3, amplitude=4.0)
         b.seed_physical(1, field)
         b.step(6)
         got0 = b.read_state(0)
         got1 = b.read_state(1)
-    want0, _ = run_plain(s2t, 128, member_params(0), 7, 10, k0=12)
+    want0, _ = run_plain(s2t, n, member_params(0), 7, 10, k0=12)
     assert np.array_equal(got0, want0)
-    with s2t.Solver(128) as s:
+    with s2t.Solver(n) as s:
         s.configure(**member_params(1))
         s.seed_physical(field)
         s.step(6)

@@ -350,7 +350,7 @@ def test_js_headless_frame_loop(s2t, tmp_path):
         assert rec["energy"] == want["energy"] and rec["enstrophy"] == want["enstrophy"]
         assert rec["time"] == want["time"]
     # the oracle's FP64 E and Z of the state each record was taken from (the plain solver's state
-    # at the last record's step, equal to the driver's by the checks above)
+    # at the last record's step, equal to the frame loop's by the checks above)
     w = s2t.from_interleaved(np.fromfile(state, np.float32), 128)
     o = Scheme(128)
     last = out["records"][-1]

@@ -656,10 +656,11 @@ def main(argv=None):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(names[dom], n), "avg_launch_us": round(avg_s * 1e6, 2),
                      "bytes_per_launch": bytes_launch,
-                     "limiter": "vector-instruction issue and latency at 2 workgroups per CU "
-                                "(LDS-limited), not HBM: 32.7 M VALU instructions per launch are "
-                                "about half its time (profiles/step_kernels_4096_forced_"
-                                "turbulence.md)"},
+                     "limiter": "latency of its chain of loads, barriers and transforms at 2 "
+                                "workgroups per CU (LDS- and register-limited), not HBM bytes: its "
+                                "traffic is 1.05x the algorithmic bytes and 32.7 M VALU "
+                                "instructions per launch are about half its time (profiles/"
+                                "step_kernels_4096_forced_turbulence.md)"},
         "kernels": kernels,
         "warmup_settle": {"steps": settle_steps, "seconds": SETTLE_S},
     }

@@ -247,6 +247,39 @@ __device__ __forceinline__ void load_pass_bases(float2 (&bs)[S], Buf tb, int off
                                (uint32_t)(off + j * NS) * 8u);
 }
 
+// One butterfly of a middle/last Stockham pass (radix R, span NS): its R inputs u (read from
+// LDS) twiddled with its bases `bi` (TwBase<R>::NB values, see load_pass_bases), then the DFT in
+// registers.
+template <int R, int DIR>
+__device__ __forceinline__ void butterfly_twiddle_dft(float2 (&u)[R], const float2* bi) {
+  float2 w[R];  // w[r] = W_{NS R}^{r k}, r = 1..R-1, k = b mod NS
+  if constexpr (R >= 8) {
+    w[1] = bi[0];
+    w[2] = bi[1];
+    w[4] = bi[2];
+    w[3] = cmul(bi[1], bi[0]);
+    w[5] = cmul(bi[2], bi[0]);
+    w[6] = cmul(bi[2], bi[1]);
+    w[7] = cmul(bi[3], cconj(bi[0]));
+    if constexpr (R == 16) {
+      w[8] = bi[3];
+      w[9] = cmul(bi[3], bi[0]);
+      w[10] = cmul(bi[3], bi[1]);
+      w[11] = cmul(bi[3], w[3]);
+      w[12] = cmul(bi[3], bi[2]);
+      w[13] = cmul(bi[4], cconj(w[3]));
+      w[14] = cmul(bi[4], cconj(bi[1]));
+      w[15] = cmul(bi[4], cconj(bi[0]));
+    }
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[r] = bi[r - 1];
+  }
+#pragma unroll
+  for (int r = 1; r < R; ++r) u[r] = cmul(u[r], DIR > 0 ? cconj(w[r]) : w[r]);
+  dft<R, DIR>(u);
+}
+
 // Middle/last Stockham pass, compute part: NB butterflies of radix R per thread, span NS, inputs
 // read from LDS, twiddled with the bases `bs` (NB x TwBase<R>::NB, see load_pass_bases), DFT in
 // registers. The results stay in u for pass_store.
@@ -366,6 +399,76 @@ __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds,
     else
       load_pass_bases<N, L::R2, L::R0 * L::R1>(bs, tb, L::TW2, t);
   });
+}
+
+// The same transform, with every Stockham exchange made in two rounds through a buffer of
+// Line<N>::LDS floats, half the LDS of the full exchange: the real parts, then the imaginary
+// parts (four barriers per exchange instead of two). The arithmetic is line_fft's, so the result
+// is the same bit for bit. For a workgroup that must hold other data in LDS across a transform.
+template <int N, int DIR>
+__device__ __forceinline__ void line_fft_half(float2 (&v)[Line<N>::PTS], float* ldsf, int t,
+                                              const float2* __restrict__ tw) {
+  using L = Line<N>;
+  constexpr int T = L::T;
+  static_assert(L::NPASS == 3, "three-pass plans only");
+  const float2* twl = tw;
+  asm volatile("" : "+s"(twl));
+  const Buf tb = make_buf(twl, (uint32_t)(L::TWSIZE * 8));
+  dft<L::R0, DIR>(v);
+  using P1 = PassBases<N, L::R1, L::R0>;
+  using P2 = PassBases<N, L::R2, L::R0 * L::R1>;
+  float2 b1[P1::SIZE];
+  load_pass_bases<N, L::R1, L::R0>(b1, tb, L::TW1, t);
+  // exchange 1: thread t's R0 outputs at t R0 + r; pass 1 reads b + r N / R1
+  float2 u1[P1::NB][L::R1];
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < L::R0; ++r) ldsf[swz(t * L::R0 + r)] = half ? v[r].y : v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < P1::NB; ++i)
+#pragma unroll
+      for (int r = 0; r < L::R1; ++r) {
+        const float x = ldsf[swz(t + i * T + r * (N / L::R1))];
+        if (half) u1[i][r].y = x;
+        else u1[i][r].x = x;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < P1::NB; ++i) butterfly_twiddle_dft<L::R1, DIR>(u1[i], b1 + i * P1::PER);
+  float2 b2[P2::SIZE];
+  load_pass_bases<N, L::R2, L::R0 * L::R1>(b2, tb, L::TW2, t);
+  // exchange 2: pass 1's outputs in Stockham order (span NS = R0); pass 2 reads b + r N / R2
+  constexpr int NS = L::R0;
+  float2 u2[P2::NB][L::R2];
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < P1::NB; ++i) {
+      const int b = t + i * T;
+      const int base = (b / NS) * NS * L::R1 + b % NS;
+#pragma unroll
+      for (int r = 0; r < L::R1; ++r) ldsf[swz(base + r * NS)] = half ? u1[i][r].y : u1[i][r].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < P2::NB; ++i)
+#pragma unroll
+      for (int r = 0; r < L::R2; ++r) {
+        const float x = ldsf[swz(t + i * T + r * (N / L::R2))];
+        if (half) u2[i][r].y = x;
+        else u2[i][r].x = x;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < P2::NB; ++i) butterfly_twiddle_dft<L::R2, DIR>(u2[i], b2 + i * P2::PER);
+#pragma unroll
+  for (int i = 0; i < P2::NB; ++i)
+#pragma unroll
+    for (int r = 0; r < L::R2; ++r) v[i + r * P2::NB] = u2[i][r];
 }
 
 // The same transform with the twiddle bases already in registers (load_bases).

@@ -639,9 +639,35 @@ __global__ void __launch_bounds__(ROWFP_GROUPS * Line<N>::T) k_rows_fp(RowArgs a
 // column kernel
 // ---------------------------------------------------------------------------------------------
 
+// Column groups of two at 4096: P_hat waits for Q_hat in LDS instead of in HBM. A column's
+// in-band row blocks of a thread (blocks 0..5 and 10..15; 6..9 are out of band for every lane)
+// take NSLOT slot blocks of T values, the two partly in-band blocks (5: t <= K - 5 T, and 10:
+// t >= N - K - 10 T) sharing one. The park (C NSLOT T float2) and a half-size exchange buffer for
+// the Q' transform (C LSTRIDE floats) fit in 78.8 KB, so two workgroups still share a CU; the
+// full-size exchanges of the other transforms overlap both.
+template <int N, int C>
+struct ColPark {
+  static constexpr bool on = N == 4096 && C == 2;
+  static constexpr int T = Line<N>::T, PTS = Line<N>::PTS, K = Geo<N>::K;
+  static constexpr int SK0 = ColBand<N>::skip_first(), SKN = ColBand<N>::skip_count();
+  static constexpr int QA = SK0 - 1, QB = SK0 + SKN;  // the partly in-band blocks
+  static constexpr int NSLOT = PTS - SKN - 1;
+  // slot block of row block q (q not out of band)
+  static __device__ __forceinline__ int slot(int q) {
+    return q < SK0 ? q : (q == QB ? QA : q - SKN - 1);
+  }
+  static constexpr int LDS2 = C * NSLOT * T + C * ColCfg<N>::LSTRIDE / 2;  // float2
+};
+static_assert(ColPark<4096, 2>::QA == 5 && ColPark<4096, 2>::QB == 10 &&
+                  ColPark<4096, 2>::NSLOT == 11 &&
+                  Geo<4096>::K - 5 * 256 < (4096 - Geo<4096>::K) - 10 * 256 &&
+                  ColCfg<4096>::LSTRIDE % 2 == 0 &&
+                  2 * (ColPark<4096, 2>::LDS2 * 8 + 16) <= 160 * 1024,
+              "4096: blocks 5 and 10 in band for disjoint lanes; two workgroups per CU");
+
 template <int N, int C>
 struct ColSmem {
-  float2 lds[C * ColCfg<N>::LSTRIDE];
+  float2 lds[ColPark<N, C>::on ? ColPark<N, C>::LDS2 : C * ColCfg<N>::LSTRIDE];
   float speed;
 };
 
@@ -675,13 +701,11 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
   const Buf bu = make_buf(a.mu + inst * a.s_mu, 3 * PLANE_B);
   const Buf bw0 = make_buf(a.w0 + inst * a.s_state, STATE_B);
   const Buf bwc = make_buf(a.wcur + inst * a.s_state, STATE_B);
-  // state leaving the stage (the right-hand side in COLS_RHS); P_hat is parked here first, each
-  // thread at the indices it later overwrites with the new state. The park costs a write and a
-  // read of the in-band P_hat (60 MB of a launch's 380 MB at 4096^2); the ways around it measured
-  // slower there: P_hat held in registers across the Q' transform needs 170 VGPRs (one
-  // workgroup per CU) or spills 34-56 at 128, a second LDS line per column leaves room for one
-  // workgroup per CU, and the field-parallel kernel (P' and Q' side by side, both results in LDS)
-  // runs one column per workgroup (128 us per launch against 104).
+  // state leaving the stage (the right-hand side in COLS_RHS). Without the LDS park (PARK) P_hat
+  // is parked here first, each thread at the indices it later overwrites with the new state: a
+  // write and a read of the in-band P_hat in HBM.
+  constexpr bool PARK = ColPark<N, C>::on;
+  using PK = ColPark<N, C>;
   const Buf bwo = MODE == COLS_RHS ? make_buf(a.rhs_out, STATE_B)
                                    : make_buf(a.wout + inst * a.s_state, STATE_B);
   const Buf bf = make_buf(FORCE ? a.mf + inst * a.s_mf : a.mf, FORCE ? PLANE_B : 0u);
@@ -691,6 +715,10 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
   const int kx = cg * C + c;
   const bool active = kx < KC;
   float2* lds = lds_all + c * LSTRIDE;
+  // PARK: the column's slot blocks (slot s of thread t at park[s T + t]) and its half-size
+  // exchange buffer, after the C columns' parks
+  float2* park = lds_all + c * PK::NSLOT * T;
+  float* ldsh = reinterpret_cast<float*>(lds_all + C * PK::NSLOT * T) + c * LSTRIDE;
   const float fkx = (float)kx;
   // per-lane byte offsets: row t of column kx in a tiled mixed plane / in the compact state
   // (row block q adds q T rows: q T / 2 row pairs, T is even)
@@ -760,23 +788,45 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
         P[q] = ld2(bz, vmp, q * ROWS_B);
       }
       line_fft<N, -1>(P, lds, t, a.tw);
+      if constexpr (PARK) {
+        __syncthreads();  // the last exchange is read (the park overlaps it)
 #pragma unroll
-      for (int q = 0; q < PTS; ++q)
-        if (!qout(q) && inband(q)) st2(bwo, P[q], vs, soff_s(q));
+        for (int q = 0; q < PTS; ++q)
+          if (!qout(q) && inband(q)) park[PK::slot(q) * T + t] = P[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < PTS; ++q)
+          if (!qout(q) && inband(q)) st2(bwo, P[q], vs, soff_s(q));
+      }
     }
     {
-      // Q_hat parks in the exchange buffer, each thread at its own indices
       float2 Q[PTS];
 #pragma unroll
       for (int i = 0; i < PTS; ++i) {
         const int q = dft_load_order<PTS>(i);
         Q[q] = ld2(bz, vmp, PLANE_B + q * ROWS_B);
       }
-      line_fft<N, -1>(Q, lds, t, a.tw);
-      __syncthreads();
+      if constexpr (PARK) {
+        // the half-size exchanges leave the park in place; then N_hat replaces P_hat in the
+        // thread's own slots (no other thread reads them: no barrier)
+        line_fft_half<N, -1>(Q, ldsh, t, a.tw);
 #pragma unroll
-      for (int q = 0; q < PTS; ++q)
-        if (!qout(q)) lds[swz(t + q * T)] = Q[q];
+        for (int q = 0; q < PTS; ++q) {
+          if (qout(q) || !inband(q)) continue;
+          const int idx = t + q * T;
+          const float fky = (float)wn(idx, N);
+          float2& pq = park[PK::slot(q) * T + t];
+          pq = a.sd ? nhat(pq, Q[q], 0.5f * (fkx + fky), 0.5f * (fkx - fky))
+                    : nhat(pq, Q[q], fkx, fky);
+        }
+      } else {
+        // Q_hat parks in the exchange buffer, each thread at its own indices
+        line_fft<N, -1>(Q, lds, t, a.tw);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PTS; ++q)
+          if (!qout(q)) lds[swz(t + q * T)] = Q[q];
+      }
     }
 
     // per-mode right-hand side and RK stage, one row block at a time (rolled, so only one
@@ -792,7 +842,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
     float2 pB = zero, w0B = zero, wcB = zero, fkB = zero;
     auto fetch = [&](int q, float2& p_, float2& w0_, float2& wc_, float2& fk_) {
       const uint32_t so = soff_s(q), v = q < PTS && inband(q) ? vs : BUF_OOB;
-      p_ = ld2(bwo, v, so);
+      if constexpr (!PARK) p_ = ld2(bwo, v, so);
       if (MODE == COLS_STEP) {
         w0_ = ld2(bw0, v, so);
         // issued at stage 0 too (reading zero): a load under a run-time condition makes the
@@ -810,9 +860,11 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
         const float k2 = fkx * fkx + fky * fky;
         if (k2 > 0.f) {
           // N_hat = i kx P_hat + i ky Q_hat; from S_hat = P_hat + Q_hat, D_hat = P_hat - Q_hat it
-          // is i ((kx + ky) S_hat + (kx - ky) D_hat) / 2
-          const float2 nh = a.sd ? nhat(pv, lds[swz(idx)], 0.5f * (fkx + fky), 0.5f * (fkx - fky))
-                                 : nhat(pv, lds[swz(idx)], fkx, fky);
+          // is i ((kx + ky) S_hat + (kx - ky) D_hat) / 2 (PARK: already in the slot)
+          const float2 nh =
+              PARK ? park[PK::slot(q) * T + t]
+                   : (a.sd ? nhat(pv, lds[swz(idx)], 0.5f * (fkx + fky), 0.5f * (fkx - fky))
+                           : nhat(pv, lds[swz(idx)], fkx, fky));
           rhs = make_float2(-nh.x, -nh.y);
           if constexpr (FORCE) rhs = cadd(rhs, fkv);
           if (ph.force_amp > 0.f)
@@ -824,8 +876,15 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
         }
       }
       st2(bwo, MODE == COLS_RHS ? rhs : wv, inb ? vs : BUF_OOB, soff_s(q));
-      // the new state replaces Q_hat in this thread's own slot of the exchange buffer
-      if constexpr (MODE == COLS_STEP) lds[swz(t + q * T)] = wv;
+      // the new state replaces Q_hat in this thread's own slot of the exchange buffer (PARK:
+      // N_hat in its slot, in band only, as a shared slot block holds the other block's lanes)
+      if constexpr (MODE == COLS_STEP) {
+        if constexpr (PARK) {
+          if (inb) park[PK::slot(q) * T + t] = wv;
+        } else {
+          lds[swz(t + q * T)] = wv;
+        }
+      }
     };
     // pairs of blocks, skipping the pairs out of band (pair pi starts at block qof(pi))
     constexpr int SK0 = ColBand<N>::skip_first(), SKN = ColBand<N>::skip_count();
@@ -854,6 +913,8 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
     if (!qout(q)) {
       if constexpr (MODE == COLS_PRIME)
         w = inband(q) ? ld2(bwc, vs, soff_s(q)) : make_float2(0.f, 0.f);
+      else if constexpr (PARK)
+        w = inband(q) ? park[PK::slot(q) * T + t] : make_float2(0.f, 0.f);
       else
         w = lds[swz(t + q * T)];
     }
@@ -922,8 +983,12 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
   }
 }
 
+// At 4096 the step kernels are held to four waves per SIMD (two workgroups per CU, 128 VGPRs), as
+// many as the LDS allows (hip-clang's second launch bound is waves per SIMD): left to itself the
+// compiler gave the pointer-forcing step 129 VGPRs, three waves.
 template <int N, int C, int MODE, bool FORCE>
-__global__ void __launch_bounds__(C * Line<N>::T) k_cols(ColArgs a) {
+__global__ void __launch_bounds__(C * Line<N>::T, (ColPark<N, C>::on && MODE == COLS_STEP ? 4 : 1))
+    k_cols(ColArgs a) {
   __shared__ ColSmem<N, C> sm;
   cols_body<N, C, MODE, FORCE>(a, (int)blockIdx.x, (int)blockIdx.y, sm);
 }

@@ -657,10 +657,10 @@ def main(argv=None):
                      "traffic": measured_traffic(names[dom], n), "avg_launch_us": round(avg_s * 1e6, 2),
                      "bytes_per_launch": bytes_launch,
                      "limiter": "latency of its chain of loads, barriers and transforms at 2 "
-                                "workgroups per CU (LDS- and register-limited), not HBM bytes: its "
-                                "traffic is 1.05x the algorithmic bytes and 32.7 M VALU "
-                                "instructions per launch are about half its time (profiles/"
-                                "step_kernels_4096_forced_turbulence.md)"},
+                                "workgroups per CU (LDS- and register-limited): its traffic is "
+                                "1.05x the algorithmic bytes and 32.7 M VALU instructions per "
+                                "launch are about half its time (profiles/step_kernels_4096_"
+                                "forced_turbulence.md)"},
         "kernels": kernels,
         "warmup_settle": {"steps": settle_steps, "seconds": SETTLE_S},
     }

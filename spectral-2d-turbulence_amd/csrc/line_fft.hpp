@@ -148,14 +148,16 @@ __device__ __forceinline__ void dft(float2* v) {
 
 // ---------------------------------------------------------------------------------------------
 // radix plans. N must be 4^a or 2*4^a (src/gpu/fft.js:20-31); the first radix is the largest,
-// so a thread holds PTS = R0 values and T = N / R0 threads cooperate on one line.
+// so a thread holds PTS = R0 values and T = N / R0 threads cooperate on one line. 128 and 256
+// take three passes with R0 = 8 rather than two with R0 = 16: their kernels are bound by one
+// thread's chain of work, and twice the threads per line halve it (a 128^2 step 79 -> 57 us).
 // ---------------------------------------------------------------------------------------------
 template <int N> struct Plan;
 template <> struct Plan<16>   { static constexpr int R0 = 4,  R1 = 4,  R2 = 1; };
 template <> struct Plan<32>   { static constexpr int R0 = 8,  R1 = 4,  R2 = 1; };
 template <> struct Plan<64>   { static constexpr int R0 = 8,  R1 = 8,  R2 = 1; };
-template <> struct Plan<128>  { static constexpr int R0 = 16, R1 = 8,  R2 = 1; };
-template <> struct Plan<256>  { static constexpr int R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Plan<128>  { static constexpr int R0 = 8,  R1 = 4,  R2 = 4; };
+template <> struct Plan<256>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 4; };
 template <> struct Plan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
 template <> struct Plan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4; };
 template <> struct Plan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8; };

@@ -132,7 +132,7 @@ template <> struct ColWidths<2048> { static constexpr int n = 3; static constexp
 template <> struct ColWidths<1024> { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
 template <> struct ColWidths<512>  { static constexpr int n = 4; static constexpr int w[4] = {1, 2, 4, 8}; };
 template <> struct ColWidths<256>  { static constexpr int n = 3; static constexpr int w[4] = {4, 8, 16, 0}; };
-template <> struct ColWidths<128>  { static constexpr int n = 2; static constexpr int w[4] = {8, 16, 0, 0}; };
+template <> struct ColWidths<128>  { static constexpr int n = 3; static constexpr int w[4] = {4, 8, 16, 0}; };
 
 // every width: at most 1024 threads and 68 KB of LDS, and the thread mapping col<CW> / row<CW>
 // covers each (column, line thread) pair once
@@ -1551,7 +1551,7 @@ struct GroupCfg {
   static constexpr int ROW_NB8 = ((N / 2 + G - 1) / G + 7) / 8 * 8;
   static constexpr int COL_NB8 = ((Geo<N>::KC + C - 1) / C + 7) / 8 * 8;
 };
-static_assert(GroupCfg<256>::C == 16 && GroupCfg<512>::C == 4 && GroupCfg<1024>::C == 4,
+static_assert(GroupCfg<256>::C == 8 && GroupCfg<512>::C == 4 && GroupCfg<1024>::C == 4,
               "grouped column workgroups use compiled column-group widths");
 
 struct GroupClass {

@@ -129,4 +129,4 @@ def test_batch_default_widths(s2t):
     compiled; a single member keeps the size's single-instance default (the first listed)."""
     sizes = [512] * 2 + [1024] + [2048] * 2 + [128] * 3
     with s2t.Batch(sizes) as b:
-        assert b.column_widths() == {512: 4, 1024: s2t.column_widths(1024)[0], 2048: 4, 128: 8}
+        assert b.column_widths() == {512: 4, 1024: s2t.column_widths(1024)[0], 2048: 4, 128: 4}

@@ -69,14 +69,15 @@ def test_bandlimited_spectrum_host_matches_oracle(s2t, n, kmin, kmax):
 
 def test_column_widths_host(s2t):
     """s2t_column_widths (host only): every supported size lists its compiled column-group widths,
-    the default first, each a workgroup of at most 1024 threads (16 values per thread); a negative
+    the default first, each a workgroup of at most 1024 threads (the line FFT's first radix values
+    per thread: 16 at 1024 to 4096, 4 at 16, else 8); a negative
     width -g is the field-parallel column kernel (g thread groups on one column), the default with
     three groups at 512 and 1024 and with two at 2048."""
     fp = {512: -3, 1024: -3, 2048: -2}
     for n in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
         w = s2t.column_widths(n)
         assert len(w) == len(set(w)) >= 1
-        per = n // (8 if n in (32, 64, 512) else 4 if n == 16 else 16)
+        per = n // (16 if n >= 1024 else 4 if n == 16 else 8)
         assert all((c >= 1 and c * per <= 1024) or (c in (-2, -3) and -c * per <= 1024) for c in w)
         assert [c for c in w if c < 0] == ([fp[n]] if n in fp else [])
         assert (w[0] < 0) == (n in fp)

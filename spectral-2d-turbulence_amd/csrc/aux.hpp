@@ -10,6 +10,10 @@ namespace s2t {
 
 void fill_u32(hipStream_t st, void* p, size_t words, uint32_t v);
 void reset_inst(hipStream_t st, Inst* in, int i0, int ni);
+// parameters, clock reset and step budget of instance i in one launch (no host copy or wait)
+void admit_inst(hipStream_t st, Inst* in, int i, const Phys& ph, int steps);
+// physical vorticity of a serving scenario (S2T_SCENARIO_*) as a full complex field
+void scenario_field(hipStream_t st, float2* out, int n, int scenario, double param);
 void vpack_full(hipStream_t st, const float2* w, float2* pa, float2* pb, int n);
 void assemble_full(hipStream_t st, const float2* z, const float2* fhat, float2* rhs,
                    const Phys& ph, uint32_t step, double dt, int n);

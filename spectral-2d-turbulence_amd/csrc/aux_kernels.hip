@@ -67,6 +67,67 @@ void reset_inst(hipStream_t st, Inst* in, int i0, int ni) {
   check_launch("k_reset_inst");
 }
 
+// A slot's admission in one launch: its parameters, a clock reset as k_reset_inst's and its step
+// budget, all from the kernel's arguments, so the host neither copies nor waits (the serving
+// loop admits requests this way, ordered behind the steps already queued on the stream)
+__global__ void k_admit_inst(Inst* in, int i, Phys ph, int steps) {
+  if (threadIdx.x != 0) return;
+  in[i].ph = ph;
+  DevState& d = in[i].st;
+  d.dt = ph.dt_max;
+  d.t = 0.0;
+  d.speed = 0.f;
+  for (int k = 0; k < DevState::SPEED_SLOTS; ++k) d.speed_bits[k] = 0u;
+  d.step = 0u;
+  d.nonfinite = 0u;
+  d.active = 1;
+  d.steps_left = steps;
+  d.cols_done = 0u;
+}
+void admit_inst(hipStream_t st, Inst* in, int i, const Phys& ph, int steps) {
+  hipLaunchKernelGGL(k_admit_inst, dim3(1), dim3(64), 0, st, in, i, ph, steps);
+  check_launch("k_admit_inst");
+}
+
+// Physical vorticity of a serving scenario on x_j = 2 pi j / N (src/scenarios.js:57-165), in FP64,
+// as the real part of a full complex field: Taylor-Green 6 cos 2x cos 2y; the Gaussian dipole
+// (cores of radius 0.3 and circulation +-12 at x = 0.6 pi, y = pi -+ 0.45); the two-vortex merger
+// (amplitude 30, radius 0.22, separation `param`, 0.7 when not positive). Blobs are cut at
+// r^2 < 25 a^2 with periodic separations (common.wgsl:46-48).
+__device__ double pdelta(double a, double b) {
+  double d = a - b;
+  if (d > M_PI) d -= 2 * M_PI;
+  else if (d < -M_PI) d += 2 * M_PI;
+  return d;
+}
+__device__ double blob(double x, double y, double cx, double cy, double a, double amp) {
+  const double dx = pdelta(x, cx), dy = pdelta(y, cy);
+  const double r2 = dx * dx + dy * dy, a2 = a * a;
+  return r2 < 25 * a2 ? amp * exp(-r2 / a2) : 0.0;
+}
+__global__ void k_scenario(float2* out, int n, int scenario, double param) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * n) return;
+  const double h = 2 * M_PI / n;
+  const double x = (double)(i % n) * h, y = (double)(i / n) * h;
+  double v = 0.0;
+  if (scenario == S2T_SCENARIO_TAYLOR_GREEN) {
+    v = 6 * cos(2 * x) * cos(2 * y);
+  } else if (scenario == S2T_SCENARIO_DIPOLE) {
+    const double a = 0.3, amp = 12 / (M_PI * a * a), cx = 0.6 * M_PI;
+    v = blob(x, y, cx, M_PI - 0.45, a, amp) + blob(x, y, cx, M_PI + 0.45, a, -amp);
+  } else if (scenario == S2T_SCENARIO_MERGER) {
+    const double d = param > 0 ? param : 0.7;
+    v = blob(x, y, M_PI - d / 2, M_PI, 0.22, 30) + blob(x, y, M_PI + d / 2, M_PI, 0.22, 30);
+  }
+  out[i] = make_float2((float)v, 0.f);
+}
+void scenario_field(hipStream_t st, float2* out, int n, int scenario, double param) {
+  hipLaunchKernelGGL(k_scenario, dim3(blocks((size_t)n * n, 256)), dim3(256), 0, st, out, n,
+                     scenario, param);
+  check_launch("k_scenario");
+}
+
 // plain streaming copy (16-byte vectors, grid-stride, four vectors in flight per thread): the
 // reference kernel whose counter readings calibrate the HBM traffic counters of the step kernels,
 // and a measure of the HBM bandwidth a plain copy reaches (about 5.5 TB/s on an MI355X at 1 GiB)

@@ -305,6 +305,27 @@ void seed_physical_slot(Solver* s, int slot, const float* field) {
   sync(s);
 }
 
+// Seeding and admission for the serving loop, enqueued on the solver's stream with no host wait
+// (behind whatever steps are queued there): the scenario built on the device, or fillSpectrum,
+// into the slot's state; then its parameters, clock and step budget in one launch.
+void seed_scenario_slot_async(Solver* s, int slot, int scenario, double param) {
+  need_full(s);
+  scenario_field(s->stream, s->full, s->n, scenario, param);
+  s->ops->fft2(s, s->full, 0);
+  s->ops->pack(s, s->full, slot_state(s, slot));
+}
+
+void seed_spectrum_slot_async(Solver* s, int slot, double k0, double amplitude, uint32_t seed) {
+  seed_spectrum(s->stream, slot_state(s, slot), s->seed_sum, s->n, k0, amplitude, seed);
+}
+
+void admit_slot_async(Solver* s, int slot, const s2t_params& p, int steps) {
+  s->params_slot[slot] = p;
+  s->phys_slot[slot] = make_phys(s->n, p);
+  admit_inst(s->stream, s->inst, slot, s->phys_slot[slot], steps);
+  s->primed_slot[slot] = 0;
+}
+
 void write_state_slot(Solver* s, int slot, const float* spectrum) {
   need_full(s);
   h2d(s, s->full, spectrum, (size_t)s->n * s->n * sizeof(float2));

@@ -42,6 +42,11 @@ void diag_slot(Solver* s, int slot, s2t_diag* out, double* spectrum, double* zon
                int phase = -1);
 // limit the slot to `steps` more steps, after which it retires itself (steps < 0: no limit)
 void set_budget_slot(Solver* s, int slot, int steps);
+// the serving loop's admission, enqueued with no host wait: seed the slot (a scenario built on the
+// device, or fillSpectrum), then set its parameters, clock and step budget and make it active
+void seed_scenario_slot_async(Solver* s, int slot, int scenario, double param);
+void seed_spectrum_slot_async(Solver* s, int slot, double k0, double amplitude, uint32_t seed);
+void admit_slot_async(Solver* s, int slot, const s2t_params& p, int steps);
 void set_clock_slot(Solver* s, int slot, double t, unsigned step);
 DevState read_clock_slot(Solver* s, int slot);
 

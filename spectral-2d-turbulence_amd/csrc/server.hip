@@ -6,6 +6,8 @@
 // chunk, idle slots exit at once); a request takes a free slot of a chunk of its size, or the
 // worker allocates a new chunk if the budget has room, freeing idle chunks of other sizes when it
 // does not. A request that fits nowhere yet waits; later requests that fit are admitted past it.
+// Admission is enqueued on the chunk's stream with no host wait (the scenario field built on the
+// device, the slot's parameters and budget set by one launch), behind the steps already queued.
 // Every chunk with active slots is stepped on its own stream. Each slot retires itself on the
 // device after exactly its budgeted steps (the step epilogue), so a result does not depend on
 // when the host polls or on which other requests share the chunk: it is bit-identical to the same
@@ -40,50 +42,6 @@ int api_bad(const std::string& m);
 static double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
-}
-
-// ---------------------------------------------------------------------------------------------
-// scenarios on the host: physical vorticity on x_j = 2 pi j / N (src/scenarios.js:57-165)
-// ---------------------------------------------------------------------------------------------
-static double periodic_delta(double a, double b) {
-  double d = a - b;
-  if (d > M_PI) d -= 2 * M_PI;
-  else if (d < -M_PI) d += 2 * M_PI;
-  return d;
-}
-
-struct Core {
-  double x, y, a, amp;
-};
-
-static void gaussian_cores(int n, const std::vector<Core>& cores, std::vector<float>& f) {
-  const double h = 2 * M_PI / n;
-  for (int r = 0; r < n; ++r)
-    for (int c = 0; c < n; ++c) {
-      double acc = 0;
-      for (const Core& v : cores) {
-        const double dx = periodic_delta(c * h, v.x), dy = periodic_delta(r * h, v.y);
-        const double r2 = dx * dx + dy * dy, a2 = v.a * v.a;
-        if (r2 < 25 * a2) acc += v.amp * std::exp(-r2 / a2);
-      }
-      f[(size_t)r * n + c] = (float)acc;
-    }
-}
-
-static void build_scenario(int n, int scenario, double param, std::vector<float>& f) {
-  f.assign((size_t)n * n, 0.f);
-  const double h = 2 * M_PI / n;
-  if (scenario == S2T_SCENARIO_TAYLOR_GREEN) {
-    for (int r = 0; r < n; ++r)
-      for (int c = 0; c < n; ++c)
-        f[(size_t)r * n + c] = (float)(6 * std::cos(2 * c * h) * std::cos(2 * r * h));
-  } else if (scenario == S2T_SCENARIO_DIPOLE) {
-    const double a = 0.3, amp = 12 / (M_PI * a * a), x = 0.6 * M_PI;
-    gaussian_cores(n, {{x, M_PI - 0.45, a, amp}, {x, M_PI + 0.45, a, -amp}}, f);
-  } else if (scenario == S2T_SCENARIO_MERGER) {
-    const double d = param > 0 ? param : 0.7;
-    gaussian_cores(n, {{M_PI - d / 2, M_PI, 0.22, 30}, {M_PI + d / 2, M_PI, 0.22, 30}}, f);
-  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -220,16 +178,13 @@ static void free_chunk(Chunk* c) {
 static void admit(Server* sv, Chunk& c, int slot, const Request& r) {
   Solver* s = c.s;
   const s2t_request& q = r.req;
-  configure_slot(s, slot, q.params);
-  if (q.scenario == S2T_SCENARIO_TURBULENCE) {
-    seed_spectrum_slot(s, slot, q.k0 > 0 ? q.k0 : 14.0, q.amplitude > 0 ? q.amplitude : 6.0,
-                       q.seed);
-  } else {
-    std::vector<float> f;
-    build_scenario(s->n, q.scenario, q.scenario_param, f);
-    seed_physical_slot(s, slot, f.data());
-  }
-  set_budget_slot(s, slot, q.steps);
+  // enqueued behind the chunk's queued steps, with no host wait: the worker goes on admitting
+  if (q.scenario == S2T_SCENARIO_TURBULENCE)
+    seed_spectrum_slot_async(s, slot, q.k0 > 0 ? q.k0 : 14.0, q.amplitude > 0 ? q.amplitude : 6.0,
+                             q.seed);
+  else
+    seed_scenario_slot_async(s, slot, q.scenario, q.scenario_param);
+  admit_slot_async(s, slot, q.params, q.steps);
   c.slot_req[slot] = (int64_t)r.id;
   c.slot_phase[slot] = s->phase;
   c.active += 1;

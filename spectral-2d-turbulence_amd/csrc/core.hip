@@ -179,7 +179,7 @@ static float2* slot_state(Solver* s, int slot, int phase = -1) {
 // ---------------------------------------------------------------------------------------------
 // lifecycle (Solver.create / build / destroy, solver.js:71-140, 614-629)
 // ---------------------------------------------------------------------------------------------
-Solver* solver_new(int n, int dye_scale, int batch, bool with_dye) {
+Solver* solver_new(int n, int dye_scale, int batch, bool with_dye, hipStream_t stream) {
   Solver* s = new Solver();
   try {
     s->n = n;
@@ -189,7 +189,12 @@ Solver* solver_new(int n, int dye_scale, int batch, bool with_dye) {
     s->batch = batch;
     s->with_dye = with_dye;
     s->ops = ops_for(n);
-    hipchk(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (stream) {
+      s->stream = stream;
+      s->own_stream = false;
+    } else {
+      hipchk(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
     const size_t st = s->ops->state_elems * batch, pl = s->ops->plane_elems * batch;
     for (int i = 0; i < 3; ++i) s->state[i] = dalloc<float2>(st, "state");
     s->mixed_u = dalloc<float2>(3 * pl, "mixed omega/u/v");
@@ -250,7 +255,7 @@ void solver_free(Solver* s) {
     (void)hipEventDestroy(s->evs[i].b);
   }
   free(s->evs);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 

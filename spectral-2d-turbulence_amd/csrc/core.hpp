@@ -24,7 +24,8 @@ std::string size_error(int n);  // "" when n is a supported grid size
 s2t_params default_params();
 std::string check_params(const s2t_params& p);  // "" when valid
 
-Solver* solver_new(int n, int dye_scale, int batch, bool with_dye);
+// stream: the caller's stream to run on (kept by the caller), or null for a stream of its own
+Solver* solver_new(int n, int dye_scale, int batch, bool with_dye, hipStream_t stream = nullptr);
 void solver_free(Solver* s);
 
 void configure_slot(Solver* s, int slot, const s2t_params& p);

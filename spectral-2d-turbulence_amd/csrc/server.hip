@@ -8,7 +8,10 @@
 // does not. A request that fits nowhere yet waits; later requests that fit are admitted past it.
 // Admission is enqueued on the chunk's stream with no host wait (the scenario field built on the
 // device, the slot's parameters and budget set by one launch), behind the steps already queued.
-// Every chunk with active slots is stepped on its own stream. Each slot retires itself on the
+// Chunks run on four streams of the server (lanes), by size: 2048 and larger, 1024, 512, and the
+// smaller sizes; the device maps a process's streams onto a few hardware queues, and with a
+// stream per chunk a large chunk's steps could wait in a shared queue behind a small chunk's
+// chain of short launches. Each slot retires itself on the
 // device after exactly its budgeted steps (the step epilogue), so a result does not depend on
 // when the host polls or on which other requests share the chunk: it is bit-identical to the same
 // request run alone. Completions record their latency and FP64 diagnostics (and, on request, the
@@ -100,7 +103,11 @@ struct Server {
   int admitted = 0;
   s2t_server_callback callback = nullptr;
   void* callback_user = nullptr;
+  hipStream_t lanes[4] = {nullptr, nullptr, nullptr, nullptr};
 };
+
+// the lane of a chunk of size n
+static int lane_of(int n) { return n >= 2048 ? 0 : n == 1024 ? 1 : n == 512 ? 2 : 3; }
 
 // Completion callback (called without sv->mu held).
 static void notify(Server* sv, uint64_t id, int status) {
@@ -137,10 +144,10 @@ static int slots_for(const Server* sv, int n) {
   return it != sv->chunk_slots.end() ? it->second : default_chunk_slots(n);
 }
 
-static Chunk* new_chunk(int n, int slots) {
+static Chunk* new_chunk(int n, int slots, hipStream_t lane) {
   Chunk* c = new Chunk();
   try {
-    c->s = solver_new(n, 1, slots, false);
+    c->s = solver_new(n, 1, slots, false, lane);
     c->slot_req.assign(slots, -1);
     c->slot_phase.assign(slots, 0);
     c->slot_fresh.assign(slots, 0);
@@ -226,7 +233,7 @@ static Chunk* place(Server* sv, int n) {
     }
     if (!fits(bytes)) return nullptr;
   }
-  Chunk* c = new_chunk(n, slots);
+  Chunk* c = new_chunk(n, slots, sv->lanes[lane_of(n)]);
   sv->cls[n].push_back(c);
   std::lock_guard<std::mutex> lk(sv->mu);
   sv->used += c->bytes;
@@ -339,22 +346,32 @@ static void worker_loop(Server* sv) {
             }
           }
       for (uint64_t id : stopped) notify(sv, id, S2T_REQ_CANCELLED);
-      // 2. step every chunk with active slots, each on its own stream (enqueued only)
+      // 2. step every chunk with active slots, each on its own stream (enqueued only). The steps
+      // are enqueued round-robin over the chunks, one step of each in turn: streams share the
+      // device's few hardware queues, and chunk after chunk a large chunk's steps waited in a
+      // shared queue behind a small chunk's whole chain of short, latency-bound launches.
       std::vector<std::pair<Chunk*, int>> ran;
+      int kmax = 0;
       for (auto& kv : sv->cls)
         for (Chunk* c : kv.second) {
           if (c->active == 0) continue;
           const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c->ms_per_step)));
           ensure_primed(c->s);
           hipchk(hipEventRecord(c->e0, c->s->stream), "hipEventRecord");
-          for (int i = 0; i < k; ++i) step_once(c->s, false);
-          hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
-          // every slot's clock in one copy, behind the steps
-          hipchk(hipMemcpyAsync(c->host_inst, c->s->inst, sizeof(Inst) * c->s->batch,
-                                hipMemcpyDeviceToHost, c->s->stream),
-                 "hipMemcpyAsync");
           ran.push_back({c, k});
+          kmax = std::max(kmax, k);
         }
+      for (int i = 0; i < kmax; ++i)
+        for (auto& pr : ran)
+          if (i < pr.second) step_once(pr.first->s, false);
+      for (auto& pr : ran) {
+        Chunk* c = pr.first;
+        hipchk(hipEventRecord(c->e1, c->s->stream), "hipEventRecord");
+        // every slot's clock in one copy, behind the steps
+        hipchk(hipMemcpyAsync(c->host_inst, c->s->inst, sizeof(Inst) * c->s->batch,
+                              hipMemcpyDeviceToHost, c->s->stream),
+               "hipMemcpyAsync");
+      }
       // 3. admission, FIFO, while the steps just enqueued run: each pending request takes a slot
       // wherever `place` finds one (its seeding is ordered behind the chunk's steps on the
       // chunk's stream); one that fits nowhere yet keeps its place in the queue
@@ -432,6 +449,8 @@ static void server_free(Server* sv) {
   if (sv->worker.joinable()) sv->worker.join();
   for (auto& kv : sv->cls)
     for (Chunk* c : kv.second) free_chunk(c);
+  for (hipStream_t l : sv->lanes)
+    if (l) (void)hipStreamDestroy(l);
   delete sv;
 }
 
@@ -476,9 +495,11 @@ int s2t_server_create(const int* sizes, const int* slots, int nclasses, double b
   Server* sv = new Server();
   S2T_API_BEGIN
   sv->budget = (size_t)budget_bytes;
+  for (hipStream_t& l : sv->lanes)
+    hipchk(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "hipStreamCreate");
   for (int i = 0; i < nclasses; ++i) {
     sv->chunk_slots[sizes[i]] = slots[i];
-    Chunk* c = new_chunk(sizes[i], slots[i]);
+    Chunk* c = new_chunk(sizes[i], slots[i], sv->lanes[lane_of(sizes[i])]);
     sv->cls[sizes[i]].push_back(c);
     sv->used += c->bytes;
   }

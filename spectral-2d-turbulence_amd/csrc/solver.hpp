@@ -49,6 +49,7 @@ struct Solver {
   int n = 0, K = 0, dye_scale = 1, m = 0;
   const Ops* ops = nullptr;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: the stream is the caller's (a serving lane), not destroyed here
 
   // device buffers
   float2* state[3] = {nullptr, nullptr, nullptr};  // compact half spectra, rotating

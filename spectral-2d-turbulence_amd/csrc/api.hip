@@ -495,6 +495,7 @@ int s2t_profile_sampled(s2t_solver* h, unsigned mask, int every) {
   if (mask) {
     sync(s);
     s->nev = 0;
+    prof_reserve(s, 256);
     for (int k = 0; k < S2T_PROF_KINDS; ++k) {
       s->prof_launches[k] = 0;
       s->prof_ms[k] = 0;

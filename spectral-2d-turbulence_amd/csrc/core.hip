@@ -34,20 +34,25 @@ static T* dalloc(size_t elems, const char* what) {
 // without the system-scope fence a record does not write back and invalidate the caches, which
 // would cost stream time and leave the next kernel's L2 cold
 // ---------------------------------------------------------------------------------------------
+// grow the profiling event list to at least `count` event pairs (creating events is host work of
+// a few microseconds each: enabling profiling reserves a list, so that it stays out of the steps)
+void prof_reserve(Solver* s, int count) {
+  if (count <= s->cap_ev) return;
+  const int cap = std::max(count, s->cap_ev * 2);
+  Solver::Ev* ev = (Solver::Ev*)realloc(s->evs, sizeof(Solver::Ev) * cap);
+  if (!ev) throw Fail(S2T_ENOMEM, "profiling event list");
+  s->evs = ev;
+  for (int i = s->cap_ev; i < cap; ++i) {
+    hipchk(hipEventCreateWithFlags(&ev[i].a, hipEventDisableSystemFence), "hipEventCreate");
+    hipchk(hipEventCreateWithFlags(&ev[i].b, hipEventDisableSystemFence), "hipEventCreate");
+    s->cap_ev = i + 1;
+  }
+}
+
 void prof_begin(Solver* s, int kind) {
   s->prof_open = ((s->prof_mask >> kind) & 1u) && !s->prof_skip;
   if (!s->prof_open) return;
-  if (s->nev == s->cap_ev) {
-    const int cap = s->cap_ev ? s->cap_ev * 2 : 256;
-    Solver::Ev* ev = (Solver::Ev*)realloc(s->evs, sizeof(Solver::Ev) * cap);
-    if (!ev) throw Fail(S2T_ENOMEM, "profiling event list");
-    for (int i = s->cap_ev; i < cap; ++i) {
-      hipchk(hipEventCreateWithFlags(&ev[i].a, hipEventDisableSystemFence), "hipEventCreate");
-      hipchk(hipEventCreateWithFlags(&ev[i].b, hipEventDisableSystemFence), "hipEventCreate");
-    }
-    s->evs = ev;
-    s->cap_ev = cap;
-  }
+  if (s->nev == s->cap_ev) prof_reserve(s, s->cap_ev ? s->cap_ev * 2 : 256);
   s->evs[s->nev].kind = kind;
   hipchk(hipEventRecord(s->evs[s->nev].a, s->stream), "hipEventRecord");
 }

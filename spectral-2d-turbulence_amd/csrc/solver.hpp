@@ -100,6 +100,7 @@ struct Solver {
 };
 
 // profiling brackets around a launch
+void prof_reserve(Solver* s, int count);
 void prof_begin(Solver* s, int kind);
 void prof_end(Solver* s);
 

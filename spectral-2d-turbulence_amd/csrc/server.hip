@@ -275,13 +275,12 @@ static void complete(Server* sv, Chunk& c, int slot) {
   sv->t_last = t;
 }
 
-// Stop a running request whose cancellation was asked for: its slot goes idle on the device
-// (no step kernel of the chunk is in flight when the worker calls this) and is freed.
+// Stop a running request whose cancellation was asked for: its slot goes idle on the device and is
+// freed. The flag is cleared on the chunk's stream, so it lands after the slot's admission (which
+// may still be queued there) and before any later step or admission of the slot.
 static void cancel_slot(Server* sv, Chunk& c, int slot) {
   const uint64_t id = (uint64_t)c.slot_req[slot];
-  const int zero = 0;
-  hipchk(hipMemcpy(&c.s->inst[slot].st.active, &zero, sizeof(int), hipMemcpyHostToDevice),
-         "hipMemcpy");
+  fill_u32(c.s->stream, &c.s->inst[slot].st.active, 1, 0u);
   c.slot_req[slot] = -1;
   c.active -= 1;
   std::lock_guard<std::mutex> lk(sv->mu);

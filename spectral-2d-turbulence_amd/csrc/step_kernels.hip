@@ -780,6 +780,14 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
           if (inband(q)) st2(bfk, F[q], vs, soff_s(q));
       }
     }
+    float2 Q[PTS];
+    auto load_q = [&] {
+#pragma unroll
+      for (int i = 0; i < PTS; ++i) {
+        const int q = dft_load_order<PTS>(i);
+        Q[q] = ld2(bz, vmp, PLANE_B + q * ROWS_B);
+      }
+    };
     {
       float2 P[PTS];
 #pragma unroll
@@ -789,6 +797,9 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
       }
       line_fft<N, -1>(P, lds, t, a.tw);
       if constexpr (PARK) {
+        // Q' is fetched while the workgroup meets at the barrier and P_hat goes to the park
+        // (0.7% of a step; not with pointer forcing, whose step kernel would spill 15 VGPRs)
+        if constexpr (!FORCE) load_q();
         __syncthreads();  // the last exchange is read (the park overlaps it)
 #pragma unroll
         for (int q = 0; q < PTS; ++q)
@@ -800,12 +811,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
       }
     }
     {
-      float2 Q[PTS];
-#pragma unroll
-      for (int i = 0; i < PTS; ++i) {
-        const int q = dft_load_order<PTS>(i);
-        Q[q] = ld2(bz, vmp, PLANE_B + q * ROWS_B);
-      }
+      if constexpr (!PARK || FORCE) load_q();
       if constexpr (PARK) {
         // the half-size exchanges leave the park in place; then N_hat replaces P_hat in the
         // thread's own slots (no other thread reads them: no barrier)

@@ -561,13 +561,18 @@ __device__ __forceinline__ float2 sample_vel4_at(Va4&& va4, int n, float px, flo
   return make_float2(a.x + (b.x - a.x) * fy, a.y + (b.y - a.y) * fy);
 }
 __device__ __forceinline__ float fract1(float x) { return x - floorf(x); }
-// RK2 midpoint back-trace (dye.wgsl:40-45) through a corner source
+// RK2 midpoint back-trace (dye.wgsl:40-45) through a corner source, from the velocity v1 at the
+// point itself (its first sample, the same for a forward and a backward trace of the point)
 template <class Va4>
-__device__ __forceinline__ float2 trace4_at(Va4&& va4, int n, float px, float py, float h) {
-  const float2 v1 = sample_vel4_at(va4, n, px, py);
+__device__ __forceinline__ float2 trace4_from(Va4&& va4, int n, float px, float py, float2 v1,
+                                              float h) {
   const float mx = fract1(px - 0.5f * v1.x * h + 1.f), my = fract1(py - 0.5f * v1.y * h + 1.f);
   const float2 v2 = sample_vel4_at(va4, n, mx, my);
   return make_float2(fract1(px - v2.x * h + 1.f), fract1(py - v2.y * h + 1.f));
+}
+template <class Va4>
+__device__ __forceinline__ float2 trace4_at(Va4&& va4, int n, float px, float py, float h) {
+  return trace4_from(va4, n, px, py, sample_vel4_at(va4, n, px, py), h);
 }
 // the same through a cell source `va(ix, iy)`
 template <class Va>
@@ -706,6 +711,12 @@ __device__ __forceinline__ float4 half3(ushort4 v) {
   return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
                      __half2float(__ushort_as_half(v.z)), 1.f);
 }
+// colour of a point staged in LDS as three FP32 planes per row: red at q[0], green at q[W], blue
+// at q[2 W] (W points per row), alpha 1
+template <int W>
+__device__ __forceinline__ float4 rgb_at(const float* q) {
+  return make_float4(q[0], q[W], q[2 * W], 1.f);
+}
 // binary16 of a colour, alpha 1
 __device__ __forceinline__ ushort4 to_half3(float4 v) {
   return make_ushort4(__half_as_ushort(__float2half(v.x)), __half_as_ushort(__float2half(v.y)),
@@ -720,7 +731,7 @@ __device__ __forceinline__ ushort4 to_half3(float4 v) {
 // for bit).
 template <int R, bool CHECKED>
 __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
-                                         const float2* __restrict__ vel, const ushort4* s0,
+                                         const float2* __restrict__ vel, const float* s0,
                                          ushort4* s1, ushort4* s2, const float2* sv,
                                          float2* sfoot, float hf,
                                          float hb, float fade, const Splats& sp, int n, int m,
@@ -752,16 +763,16 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
   };
   auto phi0 = [=](int ix, int iy) -> float4 {
     const int rx = (ix - x0 + DYE_E0) & mask, ry = (iy - y0 + DYE_E0) & mask;
-    if (!CHECKED || (rx < DYE_R0 && ry < DYE_R0)) return half3(s0[ry * DYE_R0 + rx]);
+    if (!CHECKED || (rx < DYE_R0 && ry < DYE_R0)) return rgb_at<DYE_R0>(s0 + ry * 3 * DYE_R0 + rx);
     return half3(src[(size_t)(iy & mask) * m + (ix & mask)]);
   };
   auto phi0_4 = [=](int ix, int iy, float4(&c)[4]) {
     if constexpr (!CHECKED) {
-      const ushort4* q = s0 + ((iy - y0 + DYE_E0) & mask) * DYE_R0 + ((ix - x0 + DYE_E0) & mask);
-      c[0] = half3(q[0]);
-      c[1] = half3(q[1]);
-      c[2] = half3(q[DYE_R0]);
-      c[3] = half3(q[DYE_R0 + 1]);
+      const float* q = s0 + ((iy - y0 + DYE_E0) & mask) * 3 * DYE_R0 + ((ix - x0 + DYE_E0) & mask);
+      c[0] = rgb_at<DYE_R0>(q);
+      c[1] = rgb_at<DYE_R0>(q + 1);
+      c[2] = rgb_at<DYE_R0>(q + 3 * DYE_R0);
+      c[3] = rgb_at<DYE_R0>(q + 3 * DYE_R0 + 1);
     } else {
       c[0] = phi0(ix, iy);
       c[1] = phi0(ix + 1, iy);
@@ -777,12 +788,11 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
     const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hf);
     return to_half3(sample4_at<true>(phi0_4, m, f.x, f.y));
   };
-  // the same at a point of the phi1 region, keeping the forward foot of the tile's own texels
-  // (the trace of the correction stage)
-  auto phi1_keep = [=](int i) -> ushort4 {
-    const int rx = i % DYE_R1, ry = i / DYE_R1;
+  // the same at point (rx, ry) of the phi1 region from the velocity v1 at the point, keeping the
+  // forward foot of the tile's own texels (the trace of the correction stage)
+  auto phi1_keep = [=](int rx, int ry, float2 v1) -> ushort4 {
     const int ix = x0 - DYE_E1 + rx, iy = y0 - DYE_E1 + ry;
-    const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hf);
+    const float2 f = trace4_from(va4, n, centre(ix), centre(iy), v1, hf);
     const int tx = rx - DYE_E1, ty = ry - DYE_E1;
     if (tx >= 0 && tx < DYE_T && ty >= 0 && ty < DYE_T) sfoot[ty * DYE_T + tx] = f;
     return to_half3(sample4_at<true>(phi0_4, m, f.x, f.y));
@@ -820,7 +830,6 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
     const float2 f = trace4_at(va4, n, centre(ix), centre(iy), hb);
     return to_half3(sample4_at<true>(p1, m, f.x, f.y));
   };
-  auto phih_of = [=](int ix, int iy) -> ushort4 { return phih_from(phi1_4, ix, iy); };
   auto phih = [=](int ix, int iy) -> float4 {
     const int rx = (ix - x0 + DYE_E2) & mask, ry = (iy - y0 + DYE_E2) & mask;
     if (!CHECKED || (rx < DYE_R2 && ry < DYE_R2)) return half3(s2[ry * DYE_R2 + rx]);
@@ -840,22 +849,47 @@ __device__ __forceinline__ void dye_tile(const ushort4* __restrict__ src, ushort
       c[3] = phih(ix + 1, iy + 1);
     }
   };
-  // two items per thread per iteration: independent chains the scheduler can interleave
-  for (int i = tid; i < DYE_R1 * DYE_R1; i += 2 * DYE_THREADS) {
-    const int k = i + DYE_THREADS;
-    const ushort4 a = phi1_keep(i);
-    if (k < DYE_R1 * DYE_R1) s1[k] = phi1_keep(k);
-    s1[i] = a;
+  // The phi1 stage as NALL items: items below NIN are the points of the phi_hat region (row-major,
+  // one texel in from the phi1 region's edge), the rest the ring of the phi1 region around it.
+  // Item i of a thread is point i of the phi_hat stage as well, so the velocity at the point (the
+  // first sample of both its forward and its backward trace) stays in registers between the two.
+  constexpr int NIN = DYE_R2 * DYE_R2, NALL = DYE_R1 * DYE_R1;
+  constexpr int NH = (NIN + DYE_THREADS - 1) / DYE_THREADS;   // phi_hat points per thread
+  constexpr int NI = (NALL + DYE_THREADS - 1) / DYE_THREADS;  // phi1 items per thread
+  constexpr int ED = DYE_E1 - DYE_E2;
+  static_assert(ED == 1, "the ring is one texel wide");
+  auto item = [](int i) -> int2 {
+    if (i < NIN) return make_int2(i % DYE_R2 + ED, i / DYE_R2 + ED);
+    const int r = i - NIN;
+    return r < DYE_R1       ? make_int2(r, 0)
+         : r < 2 * DYE_R1   ? make_int2(r - DYE_R1, DYE_R1 - 1)
+         : r < 2 * DYE_R1 + DYE_R2 ? make_int2(0, r - 2 * DYE_R1 + ED)
+                                   : make_int2(DYE_R1 - 1, r - 2 * DYE_R1 - DYE_R2 + ED);
+  };
+  float2 v1k[NH];
+#pragma unroll
+  for (int k = 0; k < NI; ++k) {
+    const int i = tid + k * DYE_THREADS;
+    // every thread has the first items (the compiler then interleaves their independent chains)
+    if ((k + 1) * DYE_THREADS <= NALL || i < NALL) {
+      const int2 p = item(i);
+      const float2 v1 = sample_vel4_at(va4, n, centre(x0 - DYE_E1 + p.x), centre(y0 - DYE_E1 + p.y));
+      if (k < NH) v1k[k] = v1;
+      s1[p.y * DYE_R1 + p.x] = phi1_keep(p.x, p.y, v1);
+    }
   }
   __syncthreads();
   // phi_hat into registers (all of a thread's points at once: independent chains), then over
   // phi1's LDS space once every thread is done reading phi1
-  constexpr int NH = (DYE_R2 * DYE_R2 + DYE_THREADS - 1) / DYE_THREADS;
   ushort4 hv[NH];
 #pragma unroll
   for (int k = 0; k < NH; ++k) {
     const int i = tid + k * DYE_THREADS;
-    if (i < DYE_R2 * DYE_R2) hv[k] = phih_of(x0 - DYE_E2 + i % DYE_R2, y0 - DYE_E2 + i / DYE_R2);
+    if (i < NIN) {
+      const int ix = x0 - DYE_E2 + i % DYE_R2, iy = y0 - DYE_E2 + i / DYE_R2;
+      const float2 f = trace4_from(va4, n, centre(ix), centre(iy), v1k[k], hb);
+      hv[k] = to_half3(sample4_at<true>(phi1_4, m, f.x, f.y));
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -897,7 +931,9 @@ k_dye_fused(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
             int n, int m) {
   using V = DyeVel<R>;
   static_assert(DYE_THREADS % 64 == 0, "whole waves");
-  __shared__ ushort4 s0[DYE_R0 * DYE_R0];
+  // phi0 in FP32 (converted once here rather than at each of the stencils that read it): per row,
+  // the red, green and blue planes of its DYE_R0 points
+  __shared__ float s0[3 * DYE_R0 * DYE_R0];
   // phi_hat (R2 x R2) takes phi1's space (R1 x R1) once phi1 is no longer read
   __shared__ ushort4 s1[DYE_R1 * DYE_R1];
   ushort4* s2 = s1;
@@ -927,7 +963,11 @@ k_dye_fused(const ushort4* __restrict__ src, ushort4* __restrict__ dst,
   }
   for (int i = tid; i < DYE_R0 * DYE_R0; i += DYE_THREADS) {
     const int rx = i % DYE_R0, ry = i / DYE_R0;
-    s0[i] = src[(size_t)((y0 - DYE_E0 + ry) & mask) * m + ((x0 - DYE_E0 + rx) & mask)];
+    const float4 c = half3(src[(size_t)((y0 - DYE_E0 + ry) & mask) * m + ((x0 - DYE_E0 + rx) & mask)]);
+    float* q = s0 + ry * 3 * DYE_R0 + rx;
+    q[0] = c.x;
+    q[DYE_R0] = c.y;
+    q[2 * DYE_R0] = c.z;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));

@@ -473,6 +473,68 @@ __device__ __forceinline__ void line_fft_half(float2 (&v)[Line<N>::PTS], float* 
     for (int r = 0; r < L::R2; ++r) v[i + r * P2::NB] = u2[i][r];
 }
 
+// The twiddles W^{r k}, r = 1..15, of a radix-16 butterfly from its bases (the products of
+// butterfly_twiddle_dft): w[r - 1].
+__device__ __forceinline__ void twiddles16(float2 (&w)[15], const float2* bi) {
+  w[0] = bi[0];
+  w[1] = bi[1];
+  w[3] = bi[2];
+  w[2] = cmul(bi[1], bi[0]);
+  w[4] = cmul(bi[2], bi[0]);
+  w[5] = cmul(bi[2], bi[1]);
+  w[6] = cmul(bi[3], cconj(bi[0]));
+  w[7] = bi[3];
+  w[8] = cmul(bi[3], bi[0]);
+  w[9] = cmul(bi[3], bi[1]);
+  w[10] = cmul(bi[3], w[2]);
+  w[11] = cmul(bi[3], bi[2]);
+  w[12] = cmul(bi[4], cconj(w[2]));
+  w[13] = cmul(bi[4], cconj(bi[1]));
+  w[14] = cmul(bi[4], cconj(bi[0]));
+}
+
+// The transform for plans of three radix-16 passes with one butterfly per thread in each, with
+// the first exchange's 15 twiddles of this thread (twiddles16 of its pass-1 bases) and the second
+// pass's bases already in registers: a kernel that runs several transforms forms the pass-1
+// twiddles once rather than per transform. The arithmetic is line_fft's, so the result is the same
+// bit for bit.
+template <int N, int DIR>
+__device__ __forceinline__ void line_fft_w1(float2 (&v)[Line<N>::PTS], float2* lds, int t,
+                                            const float2 (&w1)[15],
+                                            const float2 (&b2)[TwBase<16>::NB]) {
+  using L = Line<N>;
+  static_assert(L::R0 == 16 && L::R1 == 16 && L::R2 == 16 && L::T * 16 == N,
+                "three radix-16 passes, one butterfly per thread in each");
+  dft<16, DIR>(v);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) lds[swz(t * 16 + r)] = v[r];
+  __syncthreads();
+  float2 u1[1][16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) u1[0][r] = lds[swz(t + r * (N / 16))];
+#pragma unroll
+  for (int r = 1; r < 16; ++r) {
+    // opaque copies: the products stay here instead of being hoisted out of the caller's loops
+    float x = w1[r - 1].x, y = w1[r - 1].y;
+    asm volatile("" : "+v"(x), "+v"(y));
+    const float2 w = make_float2(x, y);
+    u1[0][r] = cmul(u1[0][r], DIR > 0 ? cconj(w) : w);
+  }
+  dft<16, DIR>(u1[0]);
+  float2 b[TwBase<16>::NB];
+#pragma unroll
+  for (int i = 0; i < TwBase<16>::NB; ++i) {
+    float x = b2[i].x, y = b2[i].y;
+    asm volatile("" : "+v"(x), "+v"(y));
+    b[i] = make_float2(x, y);
+  }
+  pass_store<N, 16, 16, false>(u1, v, lds, t);
+  float2 u2[1][16];
+  pass_compute<N, 16, 256, DIR>(u2, lds, t, b);
+  pass_store<N, 16, 256, true>(u2, v, lds, t);
+}
+
 // The same transform with the twiddle bases already in registers (load_bases).
 template <int N, int DIR>
 __device__ __forceinline__ void line_fft(float2 (&v)[Line<N>::PTS], float2* lds, int t,

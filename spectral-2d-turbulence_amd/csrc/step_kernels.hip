@@ -344,12 +344,24 @@ __device__ __forceinline__ void rows_body(const RowArgs& a, int bx, int iy, RowS
   constexpr bool TWREG = (KIND == ROWS_PAIRED || KIND == ROWS_PAIRED_SD) && N == 4096;
   float2 tb[Bases<N>::SIZE];
   if constexpr (TWREG) load_bases<N>(tb, a.tw, t);
+  // Stages 1-2 also form the 15 first-exchange twiddles once for the five transforms (20 more
+  // VGPRs, 158 of 168); the stage-0 kernel has no room for them.
+  constexpr bool TWFULL = KIND == ROWS_PAIRED && N == 4096;
+  float2 w1[15], b2[TwBase<16>::NB];
+  if constexpr (TWFULL) {
+    static_assert(Bases<N>::SIZE == 2 * TwBase<16>::NB, "one radix-16 butterfly per pass");
+    twiddles16(w1, tb);
+#pragma unroll
+    for (int i = 0; i < TwBase<16>::NB; ++i) b2[i] = tb[TwBase<16>::NB + i];
+  }
   auto fft_inv = [&](float2 (&x)[PTS], int tt) {
-    if constexpr (TWREG) line_fft<N, +1>(x, lds, tt, tb);
+    if constexpr (TWFULL) line_fft_w1<N, +1>(x, lds, tt, w1, b2);
+    else if constexpr (TWREG) line_fft<N, +1>(x, lds, tt, tb);
     else line_fft<N, +1>(x, lds, tt, a.tw);
   };
   auto fft_fwd = [&](float2 (&x)[PTS], int tt) {
-    if constexpr (TWREG) line_fft<N, -1>(x, lds, tt, tb);
+    if constexpr (TWFULL) line_fft_w1<N, -1>(x, lds, tt, w1, b2);
+    else if constexpr (TWREG) line_fft<N, -1>(x, lds, tt, tb);
     else line_fft<N, -1>(x, lds, tt, a.tw);
   };
 

@@ -493,18 +493,21 @@ __device__ __forceinline__ void twiddles16(float2 (&w)[15], const float2* bi) {
   w[14] = cmul(bi[4], cconj(bi[0]));
 }
 
-// The transform for plans of three radix-16 passes with one butterfly per thread in each, with
-// the first exchange's 15 twiddles of this thread (twiddles16 of its pass-1 bases) and the second
-// pass's bases already in registers: a kernel that runs several transforms forms the pass-1
-// twiddles once rather than per transform. The arithmetic is line_fft's, so the result is the same
-// bit for bit.
+// The transform for three-pass plans whose first two passes are radix 16 with one pass-1
+// butterfly per thread, with the pass-1 twiddles of this thread (twiddles16 of its pass-1 bases)
+// and the pass-2 bases already in registers: a kernel that runs several transforms forms the
+// pass-1 twiddles once rather than per transform. The arithmetic is line_fft's, so the result is
+// the same bit for bit.
+template <int N>
+using Pass2Bases = PassBases<N, Line<N>::R2, Line<N>::R0 * Line<N>::R1>;
 template <int N, int DIR>
 __device__ __forceinline__ void line_fft_w1(float2 (&v)[Line<N>::PTS], float2* lds, int t,
                                             const float2 (&w1)[15],
-                                            const float2 (&b2)[TwBase<16>::NB]) {
+                                            const float2 (&b2)[Pass2Bases<N>::SIZE]) {
   using L = Line<N>;
-  static_assert(L::R0 == 16 && L::R1 == 16 && L::R2 == 16 && L::T * 16 == N,
-                "three radix-16 passes, one butterfly per thread in each");
+  using P2 = Pass2Bases<N>;
+  static_assert(L::R0 == 16 && L::R1 == 16 && L::NPASS == 3 && L::T * 16 == N,
+                "radix-16 first passes, one pass-1 butterfly per thread");
   dft<16, DIR>(v);
   __syncthreads();
 #pragma unroll
@@ -522,17 +525,17 @@ __device__ __forceinline__ void line_fft_w1(float2 (&v)[Line<N>::PTS], float2* l
     u1[0][r] = cmul(u1[0][r], DIR > 0 ? cconj(w) : w);
   }
   dft<16, DIR>(u1[0]);
-  float2 b[TwBase<16>::NB];
+  float2 b[P2::SIZE];
 #pragma unroll
-  for (int i = 0; i < TwBase<16>::NB; ++i) {
+  for (int i = 0; i < P2::SIZE; ++i) {
     float x = b2[i].x, y = b2[i].y;
     asm volatile("" : "+v"(x), "+v"(y));
     b[i] = make_float2(x, y);
   }
   pass_store<N, 16, 16, false>(u1, v, lds, t);
-  float2 u2[1][16];
-  pass_compute<N, 16, 256, DIR>(u2, lds, t, b);
-  pass_store<N, 16, 256, true>(u2, v, lds, t);
+  float2 u2[P2::NB][L::R2];
+  pass_compute<N, L::R2, 256, DIR>(u2, lds, t, b);
+  pass_store<N, L::R2, 256, true>(u2, v, lds, t);
 }
 
 // The same transform with the twiddle bases already in registers (load_bases).

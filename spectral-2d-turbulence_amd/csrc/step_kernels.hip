@@ -339,20 +339,22 @@ __device__ __forceinline__ void rows_body(const RowArgs& a, int bx, int iy, RowS
   float* vpark = vpark_all + g * N;
   // Twiddle bases of this thread, loaded once for all transforms of the kernel: only where the
   // LDS limit (three workgroups per CU at N = 4096) leaves the registers free, in the paired
-  // variants (138 VGPRs in stages 1-2; 164 at stage 0, held to the 168 of three waves per SIMD
-  // by the launch bound, without spills); smaller N would lose occupancy.
+  // variants (164 VGPRs at stage 0, held to the 168 of three waves per SIMD by the launch bound,
+  // without spills); smaller N would lose occupancy. Stages 1-2 also form the 15 first-exchange
+  // twiddles once for their five transforms (158 VGPRs; the issue-bound kernel saves 44 VALU per
+  // transform). At 2048, where the kernel waits on its chain rather than on issue, the same took
+  // 150 VGPRs and no time off.
+  constexpr bool TWFULL = KIND == ROWS_PAIRED && N == 4096;
   constexpr bool TWREG = (KIND == ROWS_PAIRED || KIND == ROWS_PAIRED_SD) && N == 4096;
   float2 tb[Bases<N>::SIZE];
   if constexpr (TWREG) load_bases<N>(tb, a.tw, t);
-  // Stages 1-2 also form the 15 first-exchange twiddles once for the five transforms (20 more
-  // VGPRs, 158 of 168); the stage-0 kernel has no room for them.
-  constexpr bool TWFULL = KIND == ROWS_PAIRED && N == 4096;
-  float2 w1[15], b2[TwBase<16>::NB];
+  constexpr int NB2 = TWFULL ? Pass2Bases<N>::SIZE : 1;
+  float2 w1[15], b2[NB2];
   if constexpr (TWFULL) {
-    static_assert(Bases<N>::SIZE == 2 * TwBase<16>::NB, "one radix-16 butterfly per pass");
+    static_assert(Bases<N>::SIZE == TwBase<16>::NB + NB2, "one radix-16 butterfly in pass 1");
     twiddles16(w1, tb);
 #pragma unroll
-    for (int i = 0; i < TwBase<16>::NB; ++i) b2[i] = tb[TwBase<16>::NB + i];
+    for (int i = 0; i < NB2; ++i) b2[i] = tb[TwBase<16>::NB + i];
   }
   auto fft_inv = [&](float2 (&x)[PTS], int tt) {
     if constexpr (TWFULL) line_fft_w1<N, +1>(x, lds, tt, w1, b2);

@@ -73,7 +73,10 @@ struct Chunk {
   std::vector<int> slot_phase;    // chunk buffer phase when the slot's request was admitted
   std::vector<char> slot_fresh;   // admitted after this poll's clock copy was enqueued
   int active = 0;
-  double ms_per_step = 1.0;       // running estimate
+  // running estimate of the time per step, from events on the chunk's lane stream: chunks of one
+  // lane interleave their steps there, so each estimate includes the others' steps, and a lane's
+  // work per poll stays near target_ms however many chunks share it
+  double ms_per_step = 1.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   size_t bytes = 0;
   Inst* host_inst = nullptr;      // pinned copy of the slots' parameters and clocks, per poll
@@ -89,7 +92,7 @@ struct Server {
   std::thread worker;
   std::atomic<bool> stop{false};
   uint64_t next_id = 1;
-  double target_ms = 4.0;         // GPU time per chunk per poll
+  double target_ms = 4.0;         // GPU time per lane per poll (Chunk::ms_per_step)
   size_t budget = 0, used = 0, peak = 0;  // device bytes: limit (0 = none), in chunks now, most
   // statistics
   std::vector<double> latencies;
@@ -182,7 +185,9 @@ static void free_chunk(Chunk* c) {
   delete c;
 }
 
-static void admit(Server* sv, Chunk& c, int slot, const Request& r) {
+// Seed slot `slot` of chunk c for request r, which the caller has taken off the queue and marked
+// running.
+static void admit(Chunk& c, int slot, const Request& r) {
   Solver* s = c.s;
   const s2t_request& q = r.req;
   // enqueued behind the chunk's queued steps, with no host wait: the worker goes on admitting
@@ -195,10 +200,6 @@ static void admit(Server* sv, Chunk& c, int slot, const Request& r) {
   c.slot_req[slot] = (int64_t)r.id;
   c.slot_phase[slot] = s->phase;
   c.active += 1;
-  std::lock_guard<std::mutex> lk(sv->mu);
-  Record& rc = sv->rec[r.id];
-  rc.status = S2T_REQ_RUNNING;
-  rc.t_start = now_s();
 }
 
 // A chunk of size n with a free slot for a new request: an existing one, else a new one if the
@@ -389,14 +390,20 @@ static void worker_loop(Server* sv) {
         int slot = 0;
         while (c->slot_req[slot] >= 0) ++slot;
         {
+          // taken off the queue and marked running together, so that a cancellation sees either
+          // a queued request (and removes it) or a running one (and flags it for the worker)
           std::lock_guard<std::mutex> lk(sv->mu);
+          Record& rc = sv->rec[r.id];
+          if (rc.status != S2T_REQ_QUEUED) continue;  // cancelled since the check above
           for (auto p = sv->pending.begin(); p != sv->pending.end(); ++p)
             if (p->id == r.id) {
               sv->pending.erase(p);
               break;
             }
+          rc.status = S2T_REQ_RUNNING;
+          rc.t_start = now_s();
         }
-        admit(sv, *c, slot, r);
+        admit(*c, slot, r);
         c->slot_fresh[slot] = 1;
         sv->admitted += 1;
       }

@@ -177,6 +177,17 @@ struct ColBand {
     for (int q = skip_first(); q + 1 < PTS && out(q) && out(q + 1); q += 2) n += 2;
     return n;
   }
+  // the run of blocks out of band for every lane, block by block (PTS and 0 when there is none)
+  static constexpr int out_first() {
+    for (int q = 0; q < PTS; ++q)
+      if (out(q)) return q;
+    return PTS;
+  }
+  static constexpr int out_count() {
+    int n = 0;
+    for (int q = out_first(); q < PTS && out(q); ++q) ++n;
+    return n;
+  }
 };
 static_assert(ColBand<4096>::skip_first() == 6 && ColBand<4096>::skip_count() == 4,
               "4096: blocks 6..9 are out of band");
@@ -653,31 +664,49 @@ __global__ void __launch_bounds__(ROWFP_GROUPS * Line<N>::T) k_rows_fp(RowArgs a
 // column kernel
 // ---------------------------------------------------------------------------------------------
 
-// Column groups of two at 4096: P_hat waits for Q_hat in LDS instead of in HBM. A column's
-// in-band row blocks of a thread (blocks 0..5 and 10..15; 6..9 are out of band for every lane)
-// take NSLOT slot blocks of T values, the two partly in-band blocks (5: t <= K - 5 T, and 10:
-// t >= N - K - 10 T) sharing one. The park (C NSLOT T float2) and a half-size exchange buffer for
-// the Q' transform (C LSTRIDE floats) fit in 78.8 KB, so two workgroups still share a CU; the
-// full-size exchanges of the other transforms overlap both.
+// P_hat waits for Q_hat in LDS instead of in HBM: in column groups of two at 4096, in the groups
+// of the batch's grouped launches (8 columns at 256, 4 at 512 and 1024) and in groups of four at
+// 128 to 2048 (the batch's per-class default). A column's in-band
+// row blocks of a thread (at 4096 blocks 0..5 and 10..15; 6..9 are out of band for every lane)
+// take NSLOT slot blocks of T values; the two partly in-band blocks around the out-of-band run
+// (QA: t <= K - QA T, QB: t >= N - K - QB T) share one where their lanes are disjoint (4096,
+// 1024). The park (C NSLOT T float2) and a half-size exchange buffer for the Q' transform
+// (C LSTRIDE floats) hold the full-size exchanges of the other transforms, which overlap both. At
+// 4096 they fit in 78.8 KB, so two workgroups still share a CU; the grouped launches' largest
+// (1024) takes 39.4 KB, four workgroups per CU as before; four columns at 2048 take 78.8 KB, two
+// 512-thread workgroups per CU as before.
 template <int N, int C>
 struct ColPark {
-  static constexpr bool on = N == 4096 && C == 2;
+  static constexpr bool on = (N == 4096 && C == 2) || (N == 256 && C == 8) ||
+                             (C == 4 && N >= 128 && N <= 2048);
   static constexpr int T = Line<N>::T, PTS = Line<N>::PTS, K = Geo<N>::K;
-  static constexpr int SK0 = ColBand<N>::skip_first(), SKN = ColBand<N>::skip_count();
-  static constexpr int QA = SK0 - 1, QB = SK0 + SKN;  // the partly in-band blocks
-  static constexpr int NSLOT = PTS - SKN - 1;
+  static constexpr int O0 = ColBand<N>::out_first(), ON = ColBand<N>::out_count();
+  static constexpr int QA = O0 - 1, QB = O0 + ON;  // the blocks around the out-of-band run
+  static constexpr bool SHARE = K - QA * T < (N - K) - QB * T;
+  static constexpr int NSLOT = PTS - ON - (SHARE ? 1 : 0);
   // slot block of row block q (q not out of band)
   static __device__ __forceinline__ int slot(int q) {
-    return q < SK0 ? q : (q == QB ? QA : q - SKN - 1);
+    return q < O0 ? q : SHARE ? (q == QB ? QA : q - ON - 1) : q - ON;
   }
   static constexpr int LDS2 = C * NSLOT * T + C * ColCfg<N>::LSTRIDE / 2;  // float2
+  static constexpr bool fits() {
+    return !on || (ON > 0 && ColCfg<N>::LSTRIDE % 2 == 0 && Line<N>::NPASS == 3 &&
+                   C * ColCfg<N>::LSTRIDE <= LDS2);
+  }
 };
 static_assert(ColPark<4096, 2>::QA == 5 && ColPark<4096, 2>::QB == 10 &&
-                  ColPark<4096, 2>::NSLOT == 11 &&
-                  Geo<4096>::K - 5 * 256 < (4096 - Geo<4096>::K) - 10 * 256 &&
-                  ColCfg<4096>::LSTRIDE % 2 == 0 &&
+                  ColPark<4096, 2>::SHARE && ColPark<4096, 2>::NSLOT == 11 &&
                   2 * (ColPark<4096, 2>::LDS2 * 8 + 16) <= 160 * 1024,
               "4096: blocks 5 and 10 in band for disjoint lanes; two workgroups per CU");
+static_assert(ColPark<1024, 4>::SHARE && ColPark<1024, 4>::NSLOT == 11 &&
+                  !ColPark<512, 4>::SHARE && ColPark<512, 4>::NSLOT == 6 &&
+                  !ColPark<256, 8>::SHARE && ColPark<256, 8>::NSLOT == 6 &&
+                  4 * (ColPark<1024, 4>::LDS2 * 8 + 16) <= 160 * 1024,
+              "grouped classes: slot blocks; four workgroups of the largest per CU");
+static_assert(ColPark<4096, 2>::fits() && ColPark<2048, 4>::fits() && ColPark<1024, 4>::fits() &&
+                  ColPark<512, 4>::fits() && ColPark<256, 8>::fits() && ColPark<256, 4>::fits() &&
+                  ColPark<128, 4>::fits() && 2 * (ColPark<2048, 4>::LDS2 * 8 + 16) <= 160 * 1024,
+              "park and half exchange hold the full exchanges; two 2048 workgroups per CU");
 
 template <int N, int C>
 struct ColSmem {
@@ -1003,11 +1032,18 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, int bx, int iy, ColS
   }
 }
 
-// At 4096 the step kernels are held to four waves per SIMD (two workgroups per CU, 128 VGPRs), as
-// many as the LDS allows (hip-clang's second launch bound is waves per SIMD): left to itself the
-// compiler gave the pointer-forcing step 129 VGPRs, three waves.
+// With the LDS park the step kernels are held to four waves per SIMD (128 VGPRs; hip-clang's
+// second launch bound is waves per SIMD): with 512 threads (4096, 2048) two workgroups per CU, as
+// many as the LDS allows (left to itself the compiler gave the pointer-forcing step at 4096 129
+// VGPRs, three waves, which at 512 threads is one workgroup per CU). The pointer-forcing kernels
+// of 256 threads stay unbound: bound they would spill, and three waves per SIMD are still three
+// workgroups.
 template <int N, int C, int MODE, bool FORCE>
-__global__ void __launch_bounds__(C * Line<N>::T, (ColPark<N, C>::on && MODE == COLS_STEP ? 4 : 1))
+__global__ void __launch_bounds__(C * Line<N>::T,
+                                  (ColPark<N, C>::on && MODE == COLS_STEP &&
+                                           (!FORCE || C * Line<N>::T >= 512)
+                                       ? 4
+                                       : 1))
     k_cols(ColArgs a) {
   __shared__ ColSmem<N, C> sm;
   cols_body<N, C, MODE, FORCE>(a, (int)blockIdx.x, (int)blockIdx.y, sm);
@@ -1656,7 +1692,8 @@ __device__ __forceinline__ void group_cols(const GroupClass& k, int stage, int b
   cols_body<N, GroupCfg<N>::C, COLS_STEP, false>(a, bx, iy, sm);
 }
 
-__global__ void __launch_bounds__(GROUP_THREADS) k_cols_group(GroupArgs g) {
+// four waves per SIMD (the LDS allows four workgroups per CU; unbound the compiler takes 141 VGPRs)
+__global__ void __launch_bounds__(GROUP_THREADS, 4) k_cols_group(GroupArgs g) {
   __shared__ ColGroupSmem sm;
   int bx, iy;
   const GroupClass& k = group_item(g, bx, iy);

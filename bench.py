@@ -141,29 +141,42 @@ def ensemble_params(i, n):
                 force_width=2.0, cfl=0.5, dt_min=1e-5, dt_max=0.02, dye_fade=1.0, fixed_dt=0.0)
 
 
-def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True):
+def bench_ensemble(s2t, steps, warmup, plain_steps, tune=True, grouped_only=False, eager=False):
     """64 independent members (32 x 256^2, 24 x 512^2, 8 x 1024^2) advanced together by the
     batch path: grouped launches (the default; six launches per step, each over every member,
     hipGraph replay), then one launch per kernel and size class on concurrent graph branches
     (default column-group widths, and the widths the tuner picks per class), against a loop over
-    the plain single-instance path on the same GPU."""
+    the plain single-instance path on the same GPU. grouped_only: the grouped launches alone (the
+    per-class launches, the tuner and the plain loop are skipped; for profiling them). eager: the
+    launches are issued one by one rather than replayed from a captured graph (the same kernels)."""
     sizes = ENSEMBLE_SIZES
     pts = sum(n * n for n in sizes)
 
-    def timed(b):
+    def timed(b, settle=False):
         b.step(warmup)
         b.sync()
+        # untimed steps until the GPU clocks have settled, as for the default workload
+        t_settle = time.perf_counter()
+        while settle and time.perf_counter() - t_settle < SETTLE_S:
+            b.step(20)
+            b.sync()
         t0 = time.perf_counter()
         b.step(steps)
         b.sync()
         return time.perf_counter() - t0
 
     with s2t.Batch(sizes) as b:
-        b.use_graphs(True)
+        b.use_graphs(not eager)
         for i, n in enumerate(sizes):
             b.configure(i, **ensemble_params(i, n))
             b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=1000 + i)
-        el = timed(b)
+        el = timed(b, settle=True)
+        if grouped_only:
+            if not all(np.isfinite(b.diagnostics(i)["energy"]) for i in range(len(sizes))):
+                raise RuntimeError("non-finite ensemble members")
+            return {"metric": "ensemble Mpts·RK3-steps/s (grouped launches only)",
+                    "value": round(pts * steps / el / 1e6, 2), "unit": "Mpts·steps/s",
+                    "steps": steps, "warmup": warmup, "ms_per_step": round(el * 1e3 / steps, 4)}
         b.use_grouped(False)
         default_widths = b.column_widths()
         el_class = timed(b)
@@ -551,6 +564,11 @@ def main(argv=None):
                     help="step: one 4096^2 instance (default); ensemble: 64 mixed-size members; "
                          "serve: Poisson arrivals into the serving loop; decaying: 1024^2 "
                          "decaying turbulence with dye on the 2048^2 grid")
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="ensemble: the grouped launches alone (no per-class launches, tuner or "
+                         "plain loop)")
+    ap.add_argument("--eager", action="store_true",
+                    help="ensemble: issue the launches one by one instead of replaying a graph")
     ap.add_argument("--plain-steps", type=int, default=50,
                     help="ensemble: steps per member of the plain-loop baseline")
     ap.add_argument("--rates", type=str, default="",
@@ -573,7 +591,8 @@ def main(argv=None):
                           "GBps": 2 * args.copy_probe / ms / 1e6}))
         return 0
     if args.workload == "ensemble":
-        print(json.dumps(bench_ensemble(s2t, args.steps, args.warmup, args.plain_steps)))
+        print(json.dumps(bench_ensemble(s2t, args.steps, args.warmup, args.plain_steps,
+                                        grouped_only=args.grouped_only, eager=args.eager)))
         return 0
     if args.workload == "horizons":
         print(json.dumps(bench_horizons(s2t, args.out)))

@@ -409,6 +409,9 @@ def bench_decaying(s2t, steps, warmup):
                        "dye": m, "hyper_p": 4, "viscosity_rate": 6.0, "cfl": 0.5,
                        "final_dt": d["dt"], "final_energy": d["energy"]},
             "dye_step": {"bound": "hbm", "avg_us": round(avg_us[s2t.PROF_DYE], 2),
+                         "schedule": "on a stream of its own beside stages 1 and 2 of its step: "
+                                     "avg_us is its span, which includes the time it shares "
+                                     "the device with them",
                          "bytes": dye_bytes, "achieved": round(dye_gbs, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dye_gbs / HBM_PEAK_GBS, 4)},

@@ -49,18 +49,18 @@ void prof_reserve(Solver* s, int count) {
   }
 }
 
-void prof_begin(Solver* s, int kind) {
+void prof_begin(Solver* s, int kind, hipStream_t st) {
   s->prof_open = ((s->prof_mask >> kind) & 1u) && !s->prof_skip;
   if (!s->prof_open) return;
   if (s->nev == s->cap_ev) prof_reserve(s, s->cap_ev ? s->cap_ev * 2 : 256);
   s->evs[s->nev].kind = kind;
-  hipchk(hipEventRecord(s->evs[s->nev].a, s->stream), "hipEventRecord");
+  hipchk(hipEventRecord(s->evs[s->nev].a, st ? st : s->stream), "hipEventRecord");
 }
 
-void prof_end(Solver* s) {
+void prof_end(Solver* s, hipStream_t st) {
   if (!s->prof_open) return;
   s->prof_open = false;
-  hipchk(hipEventRecord(s->evs[s->nev].b, s->stream), "hipEventRecord");
+  hipchk(hipEventRecord(s->evs[s->nev].b, st ? st : s->stream), "hipEventRecord");
   s->nev += 1;
 }
 
@@ -170,6 +170,14 @@ static void need_force(Solver* s) {
   if (!s->mixed_f) s->mixed_f = dalloc<float2>(s->ops->plane_elems * s->batch, "force tiles");
   if (!s->force_k) s->force_k = dalloc<float2>(s->ops->state_elems * s->batch, "force spectrum");
 }
+static void need_dye_stream(Solver* s) {
+  if (s->dye_stream) return;
+  hipchk(hipStreamCreateWithFlags(&s->dye_stream, hipStreamNonBlocking), "hipStreamCreate");
+  // device-side ordering only: no timing, and an agent-scope release (no system-scope fence)
+  const unsigned flags = hipEventDisableTiming | hipEventDisableSystemFence;
+  hipchk(hipEventCreateWithFlags(&s->ev_cols0, flags), "hipEventCreate");
+  hipchk(hipEventCreateWithFlags(&s->ev_dye, flags), "hipEventCreate");
+}
 static void need_vel(Solver* s) {
   if (!s->vel) {
     s->vel = dalloc<float2>((size_t)s->n * s->n, "velocity");
@@ -247,6 +255,12 @@ Solver* solver_new(int n, int dye_scale, int batch, bool with_dye, hipStream_t s
 void solver_free(Solver* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->dye_stream) {
+    (void)hipStreamSynchronize(s->dye_stream);
+    (void)hipStreamDestroy(s->dye_stream);
+  }
+  if (s->ev_cols0) (void)hipEventDestroy(s->ev_cols0);
+  if (s->ev_dye) (void)hipEventDestroy(s->ev_dye);
   void* bufs[] = {s->state[0], s->state[1], s->state[2], s->mixed_u,  s->mixed_z,    s->mixed_f,
                   s->force_k,  s->vel,      s->full,     s->tw,       s->wtab,       s->inst,
                   s->diag_bins, s->diag_zonal, s->seed_sum, s->dye[0], s->dye[1], s->dye[2]};
@@ -454,22 +468,35 @@ void step_once(Solver* s, bool dye) {
   s->prof_step += 1;
   const bool force = s->splats.count > 0;
   if (force) need_force(s);
-  if (dye) need_vel(s);
+  if (dye) {
+    need_vel(s);
+    need_dye_stream(s);
+  }
   ensure_primed(s);
   const bool sd = s->ops->rows(s, 0, true, dye, force);
   s->ops->cols(s, 0, force, sd);
+  if (dye) {
+    // The BFECC dye step (encodeDye, after the step in solver.js) needs the stage-0 velocity
+    // and the step's dt, both final once the stage-0 column kernel has run: it runs on its own
+    // stream beside stages 1 and 2, which write neither (the stage-2 epilogue advances the clock
+    // but not dt), and fills the device where those launches leave it idle.
+    hipchk(hipEventRecord(s->ev_cols0, s->stream), "hipEventRecord");
+    hipchk(hipStreamWaitEvent(s->dye_stream, s->ev_cols0, 0), "hipStreamWaitEvent");
+    prof_begin(s, S2T_PROF_DYE, s->dye_stream);
+    dye_step(s->dye_stream, s->dye, s->dye_parity, s->vel, &s->inst[0].st, 0.0,
+             s->phys_slot[0].dye_fade, s->splats, s->n, s->m);
+    prof_end(s, s->dye_stream);
+    hipchk(hipEventRecord(s->ev_dye, s->dye_stream), "hipEventRecord");
+    s->dye_parity = 1 - s->dye_parity;
+  }
   s->ops->rows(s, 1, false, false, false);
   s->ops->cols(s, 1, force, false);
   s->ops->rows(s, 2, false, false, false);
   s->ops->cols(s, 2, force, false);
   s->phase = (s->phase + 1) % 3;
-  if (dye) {
-    prof_begin(s, S2T_PROF_DYE);
-    dye_step(s->stream, s->dye, s->dye_parity, s->vel, &s->inst[0].st, 0.0,
-             s->phys_slot[0].dye_fade, s->splats, s->n, s->m);
-    prof_end(s);
-    s->dye_parity = 1 - s->dye_parity;
-  }
+  // whatever comes next on the solver's stream (the next step's stage-0 velocity, reads of the
+  // dye) follows the dye step
+  if (dye) hipchk(hipStreamWaitEvent(s->stream, s->ev_dye, 0), "hipStreamWaitEvent");
   s->prof_skip = false;
 }
 

@@ -85,6 +85,12 @@ struct Solver {
   int phase = 0;                  // state[phase] holds the current spectra of all slots
   int dye_parity = 0;
   bool with_dye = false;
+  // The stepped dye runs on a stream of its own, beside stages 1 and 2 of its step: it reads the
+  // stage-0 velocity and the step's dt, which they do not write. ev_cols0 marks the stage-0 column
+  // kernel on `stream`, ev_dye the dye step on dye_stream; `stream` waits for ev_dye at the end of
+  // the step, so whatever is enqueued on it afterwards follows the dye step. Lazily.
+  hipStream_t dye_stream = nullptr;
+  hipEvent_t ev_cols0 = nullptr, ev_dye = nullptr;
 
   // profiling
   unsigned prof_mask = 0;  // kernel kinds timed with events (bit k = S2T_PROF kind k)
@@ -99,10 +105,10 @@ struct Solver {
   double prof_ms[S2T_PROF_KINDS] = {0, 0, 0, 0};
 };
 
-// profiling brackets around a launch
+// profiling brackets around a launch (on the solver's stream, or on `st`)
 void prof_reserve(Solver* s, int count);
-void prof_begin(Solver* s, int kind);
-void prof_end(Solver* s);
+void prof_begin(Solver* s, int kind, hipStream_t st = nullptr);
+void prof_end(Solver* s, hipStream_t st = nullptr);
 
 // launch-error check used by the launchers; records the error text and throws
 void check_launch(const char* what);

@@ -8,6 +8,8 @@ one full IF-RK3 step (three right-hand-side evaluations, forcing, CFL reduction)
 Prints one JSON line. `value` is Mpts*RK3-steps/s with the state resident in HBM; `roofline`
 describes the dominant kernel, timed with HIP events on the solver's stream in one timed step of
 four (all of its stages); `cpu_baseline` times the FP64 NumPy oracle step on the host cores.
+Unless --no-workloads, the line also carries the batched ensemble, the serving loop and the
+1024^2 decaying workload with dye (`ensemble`, `serve`, `decaying`).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-cpu-baseline]
     python bench.py --workload ensemble [--steps K] [--plain-steps P]
@@ -708,6 +710,10 @@ def main(argv=None):
                                   for l in srv["sweep"]],
                         "workload": "80 requests, N in {128..2048}^2, steps log-uniform "
                                     "100..5000, four scenarios, 200 GB budget"}
+        # the 1024^2 decaying workload with BFECC dye on the 2048^2 grid
+        dec = bench_decaying(s2t, 300, 20)
+        out["decaying"] = {k: dec[k] for k in ("value", "unit", "ms_per_step", "kernels_us")}
+        out["decaying"]["workload"] = dec["config"]["workload"]
     if not args.no_cpu_baseline:
         # the host's cores for the threaded variant (at most 16, the box's CPU share)
         threads = min(16, len(os.sched_getaffinity(0)))

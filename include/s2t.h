@@ -280,8 +280,10 @@ int s2t_batch_sync(s2t_batch* b);
 /* s2t_autotune for every size class of the batch, each with its own member count (replaces the
  * captured steps). */
 int s2t_batch_autotune(s2t_batch* b, int steps);
-/* Column-group width in effect for a member's size class: the tuned width, else 4 for a class of
- * two or more members where 4 is compiled, else the size's default (s2t_column_widths, first). */
+/* Column-group width in effect for a member's size class: the tuned width, else for a class of
+ * two or more members the width measured fastest for that many (4 once the class fills the chip:
+ * 2048 from four members, 1024 from eight, 512 from sixteen, 128 and 256 from two; 2 for two or
+ * three of 2048), else the size's default (s2t_column_widths, first). */
 int s2t_batch_column_width(s2t_batch* b, int member, int* width);
 
 /* ------------------------------------------------------------------------------------------ */

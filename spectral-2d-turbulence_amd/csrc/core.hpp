@@ -61,7 +61,9 @@ void step_once(Solver* s, bool dye);
 // column-group width tuning (s2t_autotune): returns the chosen width, also kept in s->cols_c
 int autotune_cols(Solver* s, int steps, int* widths, double* ms, int* count);
 void set_col_width(Solver* s, int width);  // 0: the size's default; else a compiled width
-void default_class_width(Solver* s, int members);  // 4 for classes of two or more, if compiled
+// width for `instances` instances of one size in per-class launches (0: the size's default)
+int class_width(const Solver* s, int instances);
+void default_class_width(Solver* s, int members);  // class_width for classes of two or more
 void sync(Solver* s);
 
 }  // namespace s2t

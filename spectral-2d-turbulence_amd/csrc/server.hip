@@ -176,14 +176,7 @@ static Chunk* new_chunk(int n, int slots, hipStream_t lane) {
   return c;
 }
 
-static void chunk_width(Chunk& c) {
-  const bool full = c.active == (int)c.slot_req.size() && c.slot_req.size() > 1;
-  int w[MAX_COL_WIDTHS];
-  const int nw = c.s->ops->col_widths(w);
-  c.s->cols_c = 0;
-  for (int i = 0; full && i < nw; ++i)
-    if (w[i] == 4) c.s->cols_c = 4;
-}
+static void chunk_width(Chunk& c) { c.s->cols_c = class_width(c.s, c.active); }
 
 static void free_chunk(Chunk* c) {
   if (!c) return;
@@ -364,9 +357,8 @@ static void worker_loop(Server* sv) {
       for (auto& kv : sv->cls)
         for (Chunk* c : kv.second) {
           if (c->active == 0) continue;
-          // A full chunk steps fastest with four columns per column workgroup, one with idle slots
-          // with the size's single-instance kernel (the tuner's choice for one-class batches of
-          // that many instances at 512^2 to 2048^2); every width gives the same state bit for bit
+          // the column-kernel width for the chunk's busy slots (idle slots exit at once): every
+          // width gives the same state bit for bit
           chunk_width(*c);
           const int k = std::max(1, std::min(64, (int)std::lround(sv->target_ms / c->ms_per_step)));
           ensure_primed(c->s);

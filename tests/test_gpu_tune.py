@@ -125,8 +125,13 @@ def test_batch_autotune_members_equal_plain_solvers(s2t):
 
 
 def test_batch_default_widths(s2t):
-    """A size class of two or more members defaults to 4 columns per workgroup where that width is
-    compiled; a single member keeps the size's single-instance default (the first listed)."""
-    sizes = [512] * 2 + [1024] + [2048] * 2 + [128] * 3
+    """A size class of two or more members defaults to the width measured fastest for that many
+    instances: 4 columns per workgroup once the class fills the chip (128 and 256 from two members,
+    512 from sixteen, 1024 from eight, 2048 from four), 2 for two or three of 2048, else the size's
+    single-instance default (the first listed), which a single member keeps too."""
+    sizes = [512] * 2 + [1024] * 8 + [2048] * 2 + [128] * 3 + [256]
     with s2t.Batch(sizes) as b:
-        assert b.column_widths() == {512: 4, 1024: s2t.column_widths(1024)[0], 2048: 4, 128: 4}
+        assert b.column_widths() == {512: s2t.column_widths(512)[0], 1024: 4, 2048: 2, 128: 4,
+                                     256: s2t.column_widths(256)[0]}
+    with s2t.Batch([2048] * 4 + [512] * 16) as b:
+        assert b.column_widths() == {2048: 4, 512: 4}

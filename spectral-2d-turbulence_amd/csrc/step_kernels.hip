@@ -666,9 +666,9 @@ __global__ void __launch_bounds__(ROWFP_GROUPS * Line<N>::T) k_rows_fp(RowArgs a
 
 // P_hat waits for Q_hat in LDS instead of in HBM: in column groups of two at 4096, in the groups
 // of the batch's grouped launches (8 columns at 256, 4 at 512 and 1024) and in groups of four at
-// 128 to 2048 (the batch's per-class default). A column's in-band
-// row blocks of a thread (at 4096 blocks 0..5 and 10..15; 6..9 are out of band for every lane)
-// take NSLOT slot blocks of T values; the two partly in-band blocks around the out-of-band run
+// 128 to 2048 (the batch classes' and serving chunks' width once they fill the chip). A column's
+// in-band row blocks of a thread (at 4096 blocks 0..5 and 10..15; 6..9 are out of band for every
+// lane) take NSLOT slot blocks of T values; the two partly in-band blocks around the out-of-band run
 // (QA: t <= K - QA T, QB: t >= N - K - QB T) share one where their lanes are disjoint (4096,
 // 1024). The park (C NSLOT T float2) and a half-size exchange buffer for the Q' transform
 // (C LSTRIDE floats) hold the full-size exchanges of the other transforms, which overlap both. At

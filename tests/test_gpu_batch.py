@@ -105,3 +105,23 @@ def test_batch_reseed_member_between_replays(s2t, n):
 def test_batch_rejects_bad_sizes(s2t):
     with pytest.raises(s2t.S2TError, match="must be 4\\^a or 2 \\* 4\\^a"):
         s2t.Batch([64, 96])
+
+
+@pytest.mark.parametrize("members, width", [(2, 2), (4, 4)])
+def test_2048_classes_of_several_members_match_plain(s2t, members, width):
+    """2048^2 classes step by per-class launches at the width measured fastest for their member
+    count (two columns per workgroup for two or three members, four with P_hat kept in LDS from
+    four): every member, several instances in each launch, is bit-identical to its plain solver
+    (which runs the single-instance field-parallel kernel)."""
+    steps = 3
+    sizes = [2048] * members
+    with s2t.Batch(sizes) as b:
+        assert b.column_widths() == {2048: width}
+        for i in range(members):
+            b.configure(i, **member_params(i))
+            b.seed_spectrum(i, k0=30.0, amplitude=6.0, seed=2000 + i)
+        b.step(steps)
+        got = [b.read_state(i) for i in range(members)]
+    for i in range(members):
+        want, _ = run_plain(s2t, 2048, member_params(i), 2000 + i, steps, k0=30.0)
+        assert np.array_equal(got[i], want), "member %d differs" % i

@@ -84,6 +84,30 @@ def test_ensemble_64_members(s2t):
         assert err < 1e-5
 
 
+def test_ensemble_per_class_launches(s2t):
+    """The same mix stepped by per-class launches on graph branches (the classes at their default
+    widths: four columns per workgroup with P_hat kept in LDS, many instances per launch): every
+    member equals its plain solver bit for bit."""
+    sizes = bench.ENSEMBLE_SIZES
+    steps = 4
+    params = [bench.ensemble_params(i, n) for i, n in enumerate(sizes)]
+    with s2t.Batch(sizes) as b:
+        b.use_graphs(True)
+        b.use_grouped(False)
+        assert b.column_widths() == {256: 4, 512: 4, 1024: 4}
+        for i, n in enumerate(sizes):
+            b.configure(i, **params[i])
+            b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=1000 + i)
+        b.step(steps)
+        got = [b.read_state(i) for i in range(len(sizes))]
+    for i, n in enumerate(sizes):
+        with s2t.Solver(n) as s:
+            s.configure(**params[i])
+            s.seed_spectrum(k0=14.0, amplitude=6.0, seed=1000 + i)
+            s.step(steps)
+            assert np.array_equal(got[i], s.read_state()), "member %d (%d^2) differs" % (i, n)
+
+
 SERVE_PHYS = dict(beta=5.0, drag=0.01, viscosity=1.0, hyper_p=4.0, cfl=0.5, dt_min=1e-5,
                   dt_max=0.02)
 SERVE_SLOTS = {128: 4, 256: 4, 512: 2, 1024: 2, 2048: 1}

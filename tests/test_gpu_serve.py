@@ -52,6 +52,26 @@ def test_server_matches_plain_solver(s2t):
         assert np.array_equal(got, s.read_state())
 
 
+@pytest.mark.parametrize("n, slots", [(2048, 2), (512, 16)])
+def test_chunk_width_changes_keep_results(s2t, n, slots):
+    """A chunk steps at the width measured fastest for its busy slots, chosen again at every poll
+    (2048^2: two columns per workgroup while two slots are busy; 512^2: four columns with P_hat in
+    LDS while all sixteen are, the field-parallel kernel once one has finished), so the kernels
+    change under a running request: each result still equals its plain solver bit for bit."""
+    reqs = [dict(n=n, steps=6 + 3 * (k % 4), scenario="turbulence", k0=n / 40, amplitude=5.0,
+                 seed=30 + k, keep_state=True, **PHYS) for k in range(slots)]
+    with s2t.Server({n: slots}) as sv:
+        ids = [sv.submit(**r) for r in reqs]
+        sv.drain(120)
+        got = [sv.read_state(i) for i in ids]
+    for r, g in zip(reqs, got):
+        with s2t.Solver(n) as s:
+            s.configure(**PHYS)
+            s.seed_spectrum(k0=r["k0"], amplitude=5.0, seed=r["seed"])
+            s.step(r["steps"])
+            assert np.array_equal(g, s.read_state()), r["seed"]
+
+
 def chunk_bytes(n, slots):
     """Device bytes the server charges for a chunk of `slots` instances of size n: 3 states and
     5 mixed planes per instance, 3 full fields and a state of scratch per chunk."""

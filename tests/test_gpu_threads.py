@@ -1,7 +1,8 @@
-"""Solvers stepped from several host threads at once. Every solver owns its buffers, its stream and
-(with dye) a dye stream and two events; the library keeps no other state between calls, and the
-bindings release the interpreter lock while a call runs. So two solvers stepped at the same time
-from two threads end bit for bit where each ends when stepped alone."""
+"""Solvers and batches stepped from several host threads at once. Every solver owns its buffers,
+its stream and (with dye) a dye stream and two events, every batch its members, streams and
+captured graphs; the library keeps no other state between calls, and the bindings release the
+interpreter lock while a call runs. So two solvers and a batch stepped at the same time from three
+threads end bit for bit where each ends when stepped alone."""
 
 import threading
 
@@ -34,11 +35,24 @@ def run_forced(s2t, steps):
         return s.read_state(), s.diagnostics()["energy"]
 
 
-def test_two_threads_match_alone(s2t):
+BATCH_SIZES = [256, 512, 512, 1024]
+
+
+def run_batch(s2t, steps):
+    with s2t.Batch(BATCH_SIZES) as b:
+        for i in range(len(BATCH_SIZES)):
+            b.configure(i, **FORCED)
+            b.seed_spectrum(i, k0=14.0, amplitude=6.0, seed=20 + i)
+        b.step(steps)
+        return [b.read_state(i) for i in range(len(BATCH_SIZES))]
+
+
+def test_three_threads_match_alone(s2t):
     steps = 100
-    alone = {"dye": run_dye(s2t, steps), "forced": run_forced(s2t, steps)}
+    alone = {"dye": run_dye(s2t, steps), "forced": run_forced(s2t, steps),
+             "batch": run_batch(s2t, steps)}
     out, errors = {}, []
-    start = threading.Barrier(2)
+    start = threading.Barrier(3)
 
     def worker(name, fn):
         try:
@@ -48,7 +62,8 @@ def test_two_threads_match_alone(s2t):
             errors.append((name, repr(e)))
 
     threads = [threading.Thread(target=worker, args=("dye", run_dye)),
-               threading.Thread(target=worker, args=("forced", run_forced))]
+               threading.Thread(target=worker, args=("forced", run_forced)),
+               threading.Thread(target=worker, args=("batch", run_batch))]
     for t in threads:
         t.start()
     for t in threads:
@@ -59,3 +74,4 @@ def test_two_threads_match_alone(s2t):
     assert np.array_equal(w, alone["dye"][0]) and np.array_equal(dye, alone["dye"][1])
     w, e = out["forced"]
     assert np.array_equal(w, alone["forced"][0]) and e == alone["forced"][1]
+    assert all(np.array_equal(a, b) for a, b in zip(out["batch"], alone["batch"]))
